@@ -1,0 +1,690 @@
+"""moonlab_amd -- Python access to the MI355X dense state-vector engine.
+
+Everything here is a thin ctypes layer over ``moonlab_amd/lib/libmoonlab_mi355x.so``
+(built by ``python -m moonlab_amd.build``).  The job and circuit classes mirror
+the reference's Python binding (``bindings/python/moonlab/scheduler.py``,
+``bindings/python/moonlab/qgtl.py``): same method names, same status codes,
+same exceptions.  There is no CPU execution path: without the library, or
+without a visible GPU, calls that execute raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libmoonlab_mi355x.so")
+
+# ---- status codes (scheduler.h:36-42,187; moonlab_qgtl_backend.h:77-81) ----
+MOONLAB_SCHED_OK = 0
+MOONLAB_SCHED_BAD_ARG = -501
+MOONLAB_SCHED_OOM = -502
+MOONLAB_SCHED_INTERNAL = -503
+MOONLAB_SCHED_BUFFER_TOO_SMALL = -504
+MOONLAB_SCHED_NOT_BUILT = -505
+MOONLAB_SCHED_BACKEND_NOT_FOUND = -506
+MOONLAB_SCHED_BACKEND_BUSY = -507
+MOONLAB_QGTL_OK = 0
+MOONLAB_QGTL_BAD_ARG = -301
+MOONLAB_QGTL_OOM = -302
+MOONLAB_QGTL_UNSUPPORTED = -303
+MOONLAB_QGTL_INTERNAL = -304
+
+C128 = 0
+C64 = 1
+
+
+class GateType(enum.IntEnum):
+    """moonlab_qgtl_gate_t (moonlab_qgtl_backend.h:57-72) plus the wider
+    gate_* family the engine also runs (codes 14..24)."""
+    I = 0
+    X = 1
+    Y = 2
+    Z = 3
+    H = 4
+    S = 5
+    T = 6
+    RX = 7
+    RY = 8
+    RZ = 9
+    CNOT = 10
+    CY = 11
+    CZ = 12
+    SWAP = 13
+    SDG = 14
+    TDG = 15
+    PHASE = 16
+    U3 = 17
+    CPHASE = 18
+    CRX = 19
+    CRY = 20
+    CRZ = 21
+    CY_TRUE = 22
+    TOFFOLI = 23
+    FREDKIN = 24
+
+
+class SchedulerError(RuntimeError):
+    """Any negative status from the job API."""
+
+
+class QGTLError(RuntimeError):
+    """Any negative status from the circuit API."""
+
+
+class EngineError(RuntimeError):
+    """Any negative status from the moonlab_hip_* layer."""
+
+
+GATE_DTYPE = np.dtype([("type", "<i4"), ("target", "<i4"), ("control", "<i4"),
+                       ("extra", "<i4"), ("theta", "<f8"), ("phi", "<f8"), ("lam", "<f8")])
+
+
+def gate_array(gates) -> np.ndarray:
+    """(type, target[, control[, extra[, theta[, phi[, lam]]]]]) tuples -> records."""
+    if isinstance(gates, np.ndarray) and gates.dtype == GATE_DTYPE:
+        return np.ascontiguousarray(gates)
+    rows = []
+    for g in gates:
+        g = tuple(g)
+        rows.append((int(g[0]), int(g[1]), int(g[2]) if len(g) > 2 else -1,
+                     int(g[3]) if len(g) > 3 else -1, float(g[4]) if len(g) > 4 else 0.0,
+                     float(g[5]) if len(g) > 5 else 0.0, float(g[6]) if len(g) > 6 else 0.0))
+    return np.array(rows, dtype=GATE_DTYPE)
+
+
+class _JobResults(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("total_shots", ctypes.c_int),
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
+                ("num_workers_used", ctypes.c_int),
+                ("worker_seconds", ctypes.POINTER(ctypes.c_double))]
+
+
+class _QgtlOpts(ctypes.Structure):
+    _fields_ = [("num_shots", ctypes.c_int), ("rng_seed", ctypes.c_uint64),
+                ("return_probabilities", ctypes.c_int)]
+
+
+class _QgtlResults(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("num_shots", ctypes.c_int),
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
+                ("probabilities", ctypes.POINTER(ctypes.c_double))]
+
+
+class _Backend(ctypes.Structure):
+    pass
+
+
+BACKEND_EXECUTE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.POINTER(_JobResults), ctypes.c_void_p)
+COMPLETION_HOOK_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(_JobResults),
+                                      ctypes.c_char_p, ctypes.c_void_p)
+_Backend._fields_ = [("name", ctypes.c_char_p), ("execute", BACKEND_EXECUTE_FN),
+                     ("ctx", ctypes.c_void_p), ("description", ctypes.c_char_p)]
+
+
+class _ExecOpts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("return_amplitudes", ctypes.c_int),
+                ("return_probabilities", ctypes.c_int), ("return_expect_z", ctypes.c_int),
+                ("fusion_max_qubits", ctypes.c_int), ("skip_shots", ctypes.c_int)]
+
+
+class _ExecResults(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("precision", ctypes.c_int),
+                ("total_shots", ctypes.c_int), ("num_workers_used", ctypes.c_int),
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
+                ("amplitudes", ctypes.POINTER(ctypes.c_double)),
+                ("probabilities", ctypes.POINTER(ctypes.c_double)),
+                ("expect_z", ctypes.POINTER(ctypes.c_double)),
+                ("norm", ctypes.c_double), ("device_seconds", ctypes.c_double),
+                ("wall_seconds", ctypes.c_double)]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    u32, u64 = ctypes.c_uint32, ctypes.c_uint64
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    sig = {
+        # job API
+        "moonlab_job_create": ([ip], vp),
+        "moonlab_job_free": ([vp], None),
+        "moonlab_job_add_gate": ([vp, ip, ip, ip, dp], ip),
+        "moonlab_job_set_num_shots": ([vp, ip], ip),
+        "moonlab_job_set_num_workers": ([vp, ip], ip),
+        "moonlab_job_set_rng_seed": ([vp, u64], ip),
+        "moonlab_job_num_qubits": ([vp], ip),
+        "moonlab_job_num_gates": ([vp], ip),
+        "moonlab_job_num_shots": ([vp], ip),
+        "moonlab_job_num_workers": ([vp], ip),
+        "moonlab_scheduler_run": ([vp, ctypes.POINTER(_JobResults)], ip),
+        "moonlab_scheduler_run_mpi": ([vp, ctypes.POINTER(_JobResults), vp], ip),
+        "moonlab_job_results_free": ([ctypes.POINTER(_JobResults)], None),
+        "moonlab_job_to_json": ([vp, ctypes.c_char_p, ctypes.c_size_t], ip),
+        "moonlab_register_backend": ([ctypes.POINTER(_Backend)], ip),
+        "moonlab_unregister_backend": ([ctypes.c_char_p], ip),
+        "moonlab_find_backend": ([ctypes.c_char_p], ctypes.POINTER(_Backend)),
+        "moonlab_num_backends": ([], ip),
+        "moonlab_list_backends": ([ctypes.POINTER(ctypes.c_char_p), ip], ip),
+        "moonlab_job_set_backend": ([vp, ctypes.c_char_p], ip),
+        "moonlab_job_backend": ([vp], ctypes.c_char_p),
+        "moonlab_scheduler_set_completion_hook": ([COMPLETION_HOOK_FN, vp], ip),
+        "moonlab_scheduler_set_request_context": ([ctypes.c_char_p, ctypes.c_char_p], None),
+        "moonlab_scheduler_current_tenant_id": ([], ctypes.c_char_p),
+        "moonlab_scheduler_current_request_id": ([], ctypes.c_char_p),
+        "moonlab_scheduler_fire_completion_hook": ([vp, ctypes.POINTER(_JobResults),
+                                                    ctypes.c_char_p], None),
+        # circuit API
+        "moonlab_qgtl_circuit_create": ([ip], vp),
+        "moonlab_qgtl_circuit_free": ([vp], None),
+        "moonlab_qgtl_add_gate": ([vp, ip, ip, ip, dp], ip),
+        "moonlab_qgtl_execute": ([vp, ctypes.POINTER(_QgtlOpts), ctypes.POINTER(_QgtlResults)], ip),
+        "moonlab_qgtl_results_free": ([ctypes.POINTER(_QgtlResults)], None),
+        "moonlab_qgtl_circuit_num_qubits": ([vp], ip),
+        "moonlab_qgtl_circuit_num_gates": ([vp], ip),
+        "moonlab_qgtl_circuit_serialize": ([vp, ctypes.c_char_p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_size_t)], ip),
+        "moonlab_qgtl_circuit_deserialize": ([ctypes.c_char_p, ctypes.c_size_t,
+                                              ctypes.POINTER(ctypes.c_int)], vp),
+        "moonlab_qgtl_circuit_save": ([vp, ctypes.c_char_p], ip),
+        "moonlab_qgtl_circuit_load": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)], vp),
+        # MI355X additions
+        "moonlab_mi355x_register_backend": ([], ip),
+        "moonlab_mi355x_execute_ex": ([vp, ctypes.POINTER(_ExecOpts),
+                                       ctypes.POINTER(_ExecResults)], ip),
+        "moonlab_mi355x_results_free": ([ctypes.POINTER(_ExecResults)], None),
+        "moonlab_mi355x_set_state_cache_bytes": ([u64], ip),
+        # state-vector layer
+        "moonlab_hip_device_count": ([], ip),
+        "moonlab_hip_last_error": ([], ctypes.c_char_p),
+        "moonlab_hip_set_gate_tuning": ([ip, ip], ip),
+        "moonlab_hip_state_create": ([u32, ctypes.POINTER(vp)], ip),
+        "moonlab_hip_state_create_ex": ([u32, ip, ctypes.POINTER(vp)], ip),
+        "moonlab_hip_state_free": ([vp], None),
+        "moonlab_hip_state_n_qubits": ([vp], u32),
+        "moonlab_hip_state_dim": ([vp], u64),
+        "moonlab_hip_state_precision": ([vp], ip),
+        "moonlab_hip_state_reset": ([vp], ip),
+        "moonlab_hip_apply_1q": ([vp, u32, dp], ip),
+        "moonlab_hip_apply_cnot": ([vp, u32, u32], ip),
+        "moonlab_hip_apply_2q": ([vp, u32, u32, dp], ip),
+        "moonlab_hip_apply_x": ([vp, u32], ip),
+        "moonlab_hip_apply_h": ([vp, u32], ip),
+        "moonlab_hip_apply_mcx": ([vp, u64, u32], ip),
+        "moonlab_hip_apply_mcz": ([vp, u64], ip),
+        "moonlab_hip_apply_fredkin": ([vp, u32, u32, u32], ip),
+        "moonlab_hip_apply_gate": ([vp, vp], ip),
+        "moonlab_hip_apply_gates": ([vp, vp, ip], ip),
+        "moonlab_hip_apply_diag_1q": ([vp, u32, dp], ip),
+        "moonlab_hip_apply_controlled_1q": ([vp, u64, u32, dp], ip),
+        "moonlab_hip_apply_dense": ([vp, ip, ctypes.POINTER(ctypes.c_int), dp], ip),
+        "moonlab_hip_apply_gates_fused": ([vp, vp, ip, ip], ip),
+        "moonlab_hip_prob_z": ([vp, u32, dp], ip),
+        "moonlab_hip_norm": ([vp, dp], ip),
+        "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
+        "moonlab_hip_expect_x": ([vp, u32, dp], ip),
+        "moonlab_hip_expect_y": ([vp, u32, dp], ip),
+        "moonlab_hip_correlation_zz": ([vp, u32, u32, dp], ip),
+        "moonlab_hip_probabilities": ([vp, dp], ip),
+        "moonlab_hip_inner_product": ([vp, vp, dp, dp], ip),
+        "moonlab_hip_normalize": ([vp], ip),
+        "moonlab_hip_sample": ([vp, dp, ip, u64p], ip),
+        "moonlab_hip_state_copy_to_host": ([vp, dp], ip),
+        "moonlab_hip_state_copy_from_host": ([vp, dp], ip),
+        "moonlab_hip_state_read": ([vp, u64, u64, vp], ip),
+        "moonlab_hip_state_device_ptr": ([vp], vp),
+        "moonlab_hip_state_stream": ([vp], vp),
+        "moonlab_hip_synchronize": ([vp], ip),
+        "moonlab_hip_timer_begin": ([vp, ip], ip),
+        "moonlab_hip_timer_mark": ([vp], ip),
+        "moonlab_hip_timer_elapsed": ([vp, ctypes.POINTER(ctypes.c_float), ip], ip),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+def lib() -> ctypes.CDLL:
+    """Load the engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m moonlab_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    return int(lib().moonlab_hip_device_count())
+
+
+def last_error() -> str:
+    return lib().moonlab_hip_last_error().decode()
+
+
+def _check_sched(rc: int, what: str) -> None:
+    if rc != MOONLAB_SCHED_OK:
+        raise SchedulerError(f"{what}: status {rc}")
+
+
+def _check_hip(rc: int, what: str) -> None:
+    if rc != 0:
+        raise EngineError(f"{what}: status {rc}: {last_error()}")
+
+
+# ---------------------------------------------------------------------------
+# Job API (mirrors bindings/python/moonlab/scheduler.py)
+# ---------------------------------------------------------------------------
+@dataclass
+class JobResults:
+    num_qubits: int
+    total_shots: int
+    outcomes: List[int]
+    num_workers_used: int
+    worker_seconds: List[float]
+
+
+@dataclass
+class ExecResults:
+    num_qubits: int
+    precision: int
+    outcomes: np.ndarray
+    amplitudes: Optional[np.ndarray]
+    probabilities: Optional[np.ndarray]
+    expect_z: Optional[np.ndarray]
+    norm: float
+    device_seconds: float
+    wall_seconds: float
+    num_workers_used: int
+
+
+class Job:
+    """A moonlab job (scheduler.h:45-210)."""
+
+    def __init__(self, num_qubits: int) -> None:
+        self._h = lib().moonlab_job_create(int(num_qubits))
+        if not self._h:
+            raise SchedulerError(f"moonlab_job_create({num_qubits}) failed")
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.moonlab_job_free(h)
+            self._h = None
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    @property
+    def num_qubits(self) -> int:
+        return lib().moonlab_job_num_qubits(self._h)
+
+    @property
+    def num_gates(self) -> int:
+        return lib().moonlab_job_num_gates(self._h)
+
+    @property
+    def num_shots(self) -> int:
+        return lib().moonlab_job_num_shots(self._h)
+
+    @property
+    def num_workers(self) -> int:
+        return lib().moonlab_job_num_workers(self._h)
+
+    def add_gate(self, gate: int, target: int, control: int = -1,
+                 theta: Optional[float] = None) -> "Job":
+        p = (ctypes.c_double * 1)(theta) if theta is not None else None
+        _check_sched(lib().moonlab_job_add_gate(self._h, int(gate), target, control, p), "add_gate")
+        return self
+
+    def set_num_shots(self, n: int) -> "Job":
+        _check_sched(lib().moonlab_job_set_num_shots(self._h, n), "set_num_shots")
+        return self
+
+    def set_num_workers(self, n: int) -> "Job":
+        _check_sched(lib().moonlab_job_set_num_workers(self._h, n), "set_num_workers")
+        return self
+
+    def set_rng_seed(self, seed: int) -> "Job":
+        _check_sched(lib().moonlab_job_set_rng_seed(self._h, seed), "set_rng_seed")
+        return self
+
+    def set_backend(self, name: Optional[str]) -> "Job":
+        _check_sched(lib().moonlab_job_set_backend(self._h, name.encode() if name else None),
+                     "set_backend")
+        return self
+
+    @property
+    def backend(self) -> Optional[str]:
+        b = lib().moonlab_job_backend(self._h)
+        return b.decode() if b else None
+
+    def run_raw(self):
+        """moonlab_scheduler_run: (status, results or None)."""
+        res = _JobResults()
+        rc = lib().moonlab_scheduler_run(self._h, ctypes.byref(res))
+        if rc != MOONLAB_SCHED_OK:
+            return rc, None
+        out = JobResults(res.num_qubits, res.total_shots,
+                         [res.outcomes[i] for i in range(res.total_shots)],
+                         res.num_workers_used,
+                         [res.worker_seconds[i] for i in range(res.num_workers_used)])
+        lib().moonlab_job_results_free(ctypes.byref(res))
+        return rc, out
+
+    def execute(self) -> JobResults:
+        rc, out = self.run_raw()
+        _check_sched(rc, "moonlab_scheduler_run")
+        return out
+
+    def execute_ex(self, precision: int = C128, amplitudes: bool = False,
+                   probabilities: bool = False, expect_z: bool = False,
+                   fusion_max_qubits: int = 0, skip_shots: bool = False) -> ExecResults:
+        o = _ExecOpts(precision, int(amplitudes), int(probabilities), int(expect_z),
+                      fusion_max_qubits, int(skip_shots))
+        r = _ExecResults()
+        _check_sched(lib().moonlab_mi355x_execute_ex(self._h, ctypes.byref(o), ctypes.byref(r)),
+                     "moonlab_mi355x_execute_ex")
+        n = r.num_qubits
+        dim = 1 << n
+        try:
+            outc = np.ctypeslib.as_array(r.outcomes, (r.total_shots,)).copy() \
+                if r.total_shots else np.zeros(0, np.uint64)
+            amps = np.ctypeslib.as_array(r.amplitudes, (2 * dim,)).copy().view(np.complex128) \
+                if r.amplitudes else None
+            probs = np.ctypeslib.as_array(r.probabilities, (dim,)).copy() if r.probabilities else None
+            z = np.ctypeslib.as_array(r.expect_z, (n,)).copy() if r.expect_z else None
+            return ExecResults(n, r.precision, outc, amps, probs, z, r.norm, r.device_seconds,
+                               r.wall_seconds, r.num_workers_used)
+        finally:
+            lib().moonlab_mi355x_results_free(ctypes.byref(r))
+
+    def to_json(self) -> str:
+        need = lib().moonlab_job_to_json(self._h, None, 0)
+        if need < 0:
+            raise SchedulerError(f"to_json: status {need}")
+        buf = ctypes.create_string_buffer(need + 1)
+        lib().moonlab_job_to_json(self._h, buf, need + 1)
+        return buf.value.decode()
+
+
+def list_backends() -> List[str]:
+    n = lib().moonlab_num_backends()
+    arr = (ctypes.c_char_p * max(n, 1))()
+    got = lib().moonlab_list_backends(arr, n)
+    return [arr[i].decode() for i in range(got)]
+
+
+def register_mi355x_backend() -> None:
+    _check_sched(lib().moonlab_mi355x_register_backend(), "moonlab_mi355x_register_backend")
+
+
+# ---------------------------------------------------------------------------
+# Circuit API (mirrors bindings/python/moonlab/qgtl.py)
+# ---------------------------------------------------------------------------
+@dataclass
+class CircuitResults:
+    num_qubits: int
+    outcomes: np.ndarray
+    probabilities: Optional[np.ndarray]
+
+
+class Circuit:
+    def __init__(self, num_qubits: int = 0, _handle=None) -> None:
+        self._h = _handle if _handle is not None else lib().moonlab_qgtl_circuit_create(int(num_qubits))
+        if not self._h:
+            raise QGTLError(f"moonlab_qgtl_circuit_create({num_qubits}) failed")
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.moonlab_qgtl_circuit_free(h)
+            self._h = None
+
+    @property
+    def num_qubits(self) -> int:
+        return lib().moonlab_qgtl_circuit_num_qubits(self._h)
+
+    @property
+    def num_gates(self) -> int:
+        return lib().moonlab_qgtl_circuit_num_gates(self._h)
+
+    def add_gate_raw(self, gate: int, target: int, control: int = -1,
+                     theta: Optional[float] = None) -> int:
+        p = (ctypes.c_double * 1)(theta) if theta is not None else None
+        return lib().moonlab_qgtl_add_gate(self._h, int(gate), target, control, p)
+
+    def add_gate(self, gate: int, target: int, control: int = -1,
+                 theta: Optional[float] = None) -> "Circuit":
+        rc = self.add_gate_raw(gate, target, control, theta)
+        if rc:
+            raise QGTLError(f"add_gate: status {rc}")
+        return self
+
+    def execute(self, num_shots: int = 0, rng_seed: int = 0,
+                return_probabilities: bool = False) -> CircuitResults:
+        o = _QgtlOpts(num_shots, rng_seed, int(return_probabilities))
+        r = _QgtlResults()
+        rc = lib().moonlab_qgtl_execute(self._h, ctypes.byref(o), ctypes.byref(r))
+        if rc:
+            raise QGTLError(f"moonlab_qgtl_execute: status {rc}")
+        try:
+            n = r.num_qubits
+            outc = np.ctypeslib.as_array(r.outcomes, (r.num_shots,)).copy() \
+                if r.outcomes else np.zeros(0, np.uint64)
+            probs = np.ctypeslib.as_array(r.probabilities, (1 << n,)).copy() \
+                if r.probabilities else None
+            return CircuitResults(n, outc, probs)
+        finally:
+            lib().moonlab_qgtl_results_free(ctypes.byref(r))
+
+    def serialize(self) -> str:
+        need = ctypes.c_size_t(0)
+        rc = lib().moonlab_qgtl_circuit_serialize(self._h, None, 0, ctypes.byref(need))
+        if rc:
+            raise QGTLError(f"serialize: status {rc}")
+        buf = ctypes.create_string_buffer(need.value + 1)
+        rc = lib().moonlab_qgtl_circuit_serialize(self._h, buf, need.value + 1, None)
+        if rc:
+            raise QGTLError(f"serialize: status {rc}")
+        return buf.value.decode()
+
+    @classmethod
+    def deserialize(cls, text: str) -> "Circuit":
+        st = ctypes.c_int(0)
+        data = text.encode()
+        h = lib().moonlab_qgtl_circuit_deserialize(data, len(data), ctypes.byref(st))
+        if not h:
+            raise QGTLError(f"deserialize: status {st.value}")
+        return cls(_handle=h)
+
+
+# ---------------------------------------------------------------------------
+# State-vector layer (moonlab_hip_*)
+# ---------------------------------------------------------------------------
+class StateVector:
+    """A device-resident state of 2^n amplitudes (complex128 or complex64)."""
+
+    def __init__(self, num_qubits: int, precision: int = C128) -> None:
+        h = ctypes.c_void_p()
+        _check_hip(lib().moonlab_hip_state_create_ex(num_qubits, precision, ctypes.byref(h)),
+                   "state_create")
+        self._h = h.value
+        self.n = num_qubits
+        self.precision = precision
+
+    def free(self) -> None:
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.moonlab_hip_state_free(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        self.free()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def reset(self) -> "StateVector":
+        _check_hip(lib().moonlab_hip_state_reset(self._h), "reset")
+        return self
+
+    def apply(self, gates, fuse: int = 0) -> "StateVector":
+        g = gate_array(gates)
+        if len(g):
+            if fuse:
+                _check_hip(lib().moonlab_hip_apply_gates_fused(self._h, g.ctypes.data, len(g), fuse),
+                           "apply_gates_fused")
+            else:
+                _check_hip(lib().moonlab_hip_apply_gates(self._h, g.ctypes.data, len(g)), "apply_gates")
+        return self
+
+    def apply_1q(self, q: int, m) -> "StateVector":
+        m = np.ascontiguousarray(np.asarray(m, np.complex128).reshape(2, 2)).view(np.float64)
+        _check_hip(lib().moonlab_hip_apply_1q(self._h, q, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                   "apply_1q")
+        return self
+
+    def apply_2q(self, q0: int, q1: int, m) -> "StateVector":
+        m = np.ascontiguousarray(np.asarray(m, np.complex128).reshape(4, 4)).view(np.float64)
+        _check_hip(lib().moonlab_hip_apply_2q(self._h, q0, q1,
+                                              m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "apply_2q")
+        return self
+
+    def apply_dense(self, qubits: Sequence[int], m) -> "StateVector":
+        k = len(qubits)
+        m = np.ascontiguousarray(np.asarray(m, np.complex128).reshape(1 << k, 1 << k)).view(np.float64)
+        q = (ctypes.c_int * k)(*qubits)
+        _check_hip(lib().moonlab_hip_apply_dense(self._h, k, q,
+                                                 m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                   "apply_dense")
+        return self
+
+    def apply_mcx(self, control_mask: int, target: int) -> "StateVector":
+        _check_hip(lib().moonlab_hip_apply_mcx(self._h, control_mask, target), "apply_mcx")
+        return self
+
+    def apply_mcz(self, mask: int) -> "StateVector":
+        _check_hip(lib().moonlab_hip_apply_mcz(self._h, mask), "apply_mcz")
+        return self
+
+    def apply_controlled_1q(self, control_mask: int, target: int, m) -> "StateVector":
+        m = np.ascontiguousarray(np.asarray(m, np.complex128).reshape(2, 2)).view(np.float64)
+        _check_hip(lib().moonlab_hip_apply_controlled_1q(
+            self._h, control_mask, target, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+            "apply_controlled_1q")
+        return self
+
+    def synchronize(self) -> None:
+        _check_hip(lib().moonlab_hip_synchronize(self._h), "synchronize")
+
+    def amplitudes(self) -> np.ndarray:
+        out = np.empty(1 << self.n, dtype=np.complex128)
+        _check_hip(lib().moonlab_hip_state_copy_to_host(
+            self._h, out.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "copy_to_host")
+        return out
+
+    def load(self, amps: np.ndarray) -> "StateVector":
+        a = np.ascontiguousarray(amps, dtype=np.complex128)
+        assert a.size == 1 << self.n
+        _check_hip(lib().moonlab_hip_state_copy_from_host(
+            self._h, a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "copy_from_host")
+        return self
+
+    def read(self, offset: int, count: int) -> np.ndarray:
+        dt = np.complex64 if self.precision == C64 else np.complex128
+        out = np.empty(count, dtype=dt)
+        _check_hip(lib().moonlab_hip_state_read(self._h, offset, count, out.ctypes.data), "state_read")
+        return out
+
+    def expect_z_all(self):
+        z = np.empty(self.n, dtype=np.float64)
+        nrm = ctypes.c_double(0.0)
+        _check_hip(lib().moonlab_hip_expect_z_all(
+            self._h, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(nrm)), "expect_z_all")
+        return z, nrm.value
+
+    def prob_one(self, q: int) -> float:
+        v = ctypes.c_double()
+        _check_hip(lib().moonlab_hip_prob_z(self._h, q, ctypes.byref(v)), "prob_z")
+        return v.value
+
+    def norm(self) -> float:
+        v = ctypes.c_double()
+        _check_hip(lib().moonlab_hip_norm(self._h, ctypes.byref(v)), "norm")
+        return v.value
+
+    def expect_x(self, q: int) -> float:
+        v = ctypes.c_double()
+        _check_hip(lib().moonlab_hip_expect_x(self._h, q, ctypes.byref(v)), "expect_x")
+        return v.value
+
+    def expect_y(self, q: int) -> float:
+        v = ctypes.c_double()
+        _check_hip(lib().moonlab_hip_expect_y(self._h, q, ctypes.byref(v)), "expect_y")
+        return v.value
+
+    def correlation_zz(self, q1: int, q2: int) -> float:
+        v = ctypes.c_double()
+        _check_hip(lib().moonlab_hip_correlation_zz(self._h, q1, q2, ctypes.byref(v)), "correlation_zz")
+        return v.value
+
+    def probabilities(self) -> np.ndarray:
+        out = np.empty(1 << self.n, dtype=np.float64)
+        _check_hip(lib().moonlab_hip_probabilities(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "probabilities")
+        return out
+
+    def inner(self, other: "StateVector") -> complex:
+        re, im = ctypes.c_double(), ctypes.c_double()
+        _check_hip(lib().moonlab_hip_inner_product(self._h, other._h, ctypes.byref(re), ctypes.byref(im)),
+                   "inner_product")
+        return complex(re.value, im.value)
+
+    def normalize(self) -> "StateVector":
+        _check_hip(lib().moonlab_hip_normalize(self._h), "normalize")
+        return self
+
+    def sample(self, u) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(u.size, dtype=np.uint64)
+        _check_hip(lib().moonlab_hip_sample(self._h, u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            u.size, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))),
+                   "sample")
+        return out
+
+    # stream timing
+    def timer_begin(self, capacity: int) -> None:
+        _check_hip(lib().moonlab_hip_timer_begin(self._h, capacity), "timer_begin")
+
+    def timer_mark(self) -> None:
+        _check_hip(lib().moonlab_hip_timer_mark(self._h), "timer_mark")
+
+    def timer_elapsed(self, max_gaps: int) -> np.ndarray:
+        out = np.zeros(max_gaps, dtype=np.float32)
+        n = lib().moonlab_hip_timer_elapsed(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                            max_gaps)
+        if n < 0:
+            raise EngineError(f"timer_elapsed failed: {last_error()}")
+        return out[:n]
+
+
+def set_gate_tuning(pairs_per_thread: int = 0, block_threads: int = 0) -> None:
+    _check_hip(lib().moonlab_hip_set_gate_tuning(pairs_per_thread, block_threads), "set_gate_tuning")

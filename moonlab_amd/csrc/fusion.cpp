@@ -1,0 +1,252 @@
+// fusion.cpp -- k-qubit gate fusion for the MI355X engine.
+//
+// The reference fuses only runs of single-qubit gates on one qubit into a
+// 2x2 and treats every 2-qubit gate as a barrier (src/optimization/fusion/
+// fusion.c:247-292).  Here any run of gates whose qubits fit in k <= 5 is
+// folded into one 2^k x 2^k unitary composed in FP64 on the host, then
+// applied in ONE pass over the state, so a block of g gates costs one HBM
+// read + write instead of g.
+#include "fusion.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "sv_internal.h"
+
+namespace mlx {
+
+hipError_t launch_dense(const SvDev &s, int k, const int *qubits, const double *mat);
+
+namespace {
+
+// What a gate does on its qubits: a 2x2 on `target` where every control is 1,
+// or an exchange of two targets where every control is 1.
+struct GateAction {
+    int  nctrl;
+    int  ctrl[2];
+    int  target;
+    bool is_swap;
+    int  target2;
+    cd   u[4];
+};
+
+GateAction action_of(const moonlab_hip_gate_t &g)
+{
+    GateAction a{};
+    a.target = g.target;
+    const cd I(0.0, 1.0);
+    auto set = [&](cd u00, cd u01, cd u10, cd u11) {
+        a.u[0] = u00; a.u[1] = u01; a.u[2] = u10; a.u[3] = u11;
+    };
+    const double c = cos(g.theta / 2.0), s = sin(g.theta / 2.0);
+    switch (g.type) {
+    case MOONLAB_HIP_GATE_I: set(1, 0, 0, 1); break;
+    case MOONLAB_HIP_GATE_X: set(0, 1, 1, 0); break;
+    case MOONLAB_HIP_GATE_Y: set(0, -I, I, 0); break;
+    case MOONLAB_HIP_GATE_Z: set(1, 0, 0, -1); break;
+    case MOONLAB_HIP_GATE_H: {
+        const double r = 0.70710678118654752440;
+        set(r, r, r, -r);
+        break;
+    }
+    case MOONLAB_HIP_GATE_S: set(1, 0, 0, I); break;
+    case MOONLAB_HIP_GATE_SDG: set(1, 0, 0, -I); break;
+    case MOONLAB_HIP_GATE_T: set(1, 0, 0, cd(cos(M_PI / 4.0), sin(M_PI / 4.0))); break;
+    case MOONLAB_HIP_GATE_TDG: set(1, 0, 0, cd(cos(-M_PI / 4.0), sin(-M_PI / 4.0))); break;
+    case MOONLAB_HIP_GATE_PHASE: set(1, 0, 0, cd(cos(g.theta), sin(g.theta))); break;
+    case MOONLAB_HIP_GATE_RX: set(c, cd(0, -s), cd(0, -s), c); break;
+    case MOONLAB_HIP_GATE_RY: set(c, -s, s, c); break;
+    case MOONLAB_HIP_GATE_RZ: set(cd(c, -s), 0, 0, cd(c, s)); break;
+    case MOONLAB_HIP_GATE_U3:
+        set(c, -cd(cos(g.lam), sin(g.lam)) * s, cd(cos(g.phi), sin(g.phi)) * s,
+            cd(cos(g.phi + g.lam), sin(g.phi + g.lam)) * c);
+        break;
+    default: break;
+    }
+    switch (g.type) {
+    case MOONLAB_HIP_GATE_CNOT: a.nctrl = 1; a.ctrl[0] = g.control; set(0, 1, 1, 0); break;
+    case MOONLAB_HIP_GATE_CY: a.nctrl = 1; a.ctrl[0] = g.control; set(0, I, -I, 0); break;
+    case MOONLAB_HIP_GATE_CY_TRUE: a.nctrl = 1; a.ctrl[0] = g.control; set(0, -I, I, 0); break;
+    case MOONLAB_HIP_GATE_CZ: a.nctrl = 1; a.ctrl[0] = g.control; set(1, 0, 0, -1); break;
+    case MOONLAB_HIP_GATE_CPHASE:
+        a.nctrl = 1; a.ctrl[0] = g.control; set(1, 0, 0, cd(cos(g.theta), sin(g.theta)));
+        break;
+    case MOONLAB_HIP_GATE_CRX: a.nctrl = 1; a.ctrl[0] = g.control; set(c, cd(0, -s), cd(0, -s), c); break;
+    case MOONLAB_HIP_GATE_CRY: a.nctrl = 1; a.ctrl[0] = g.control; set(c, -s, s, c); break;
+    case MOONLAB_HIP_GATE_CRZ: a.nctrl = 1; a.ctrl[0] = g.control; set(cd(c, -s), 0, 0, cd(c, s)); break;
+    case MOONLAB_HIP_GATE_TOFFOLI:
+        a.nctrl = 2; a.ctrl[0] = g.control; a.ctrl[1] = g.extra; set(0, 1, 1, 0);
+        break;
+    case MOONLAB_HIP_GATE_SWAP: a.is_swap = true; a.target = g.control; a.target2 = g.target; break;
+    case MOONLAB_HIP_GATE_FREDKIN:
+        a.nctrl = 1; a.ctrl[0] = g.control; a.is_swap = true; a.target2 = g.extra;
+        break;
+    default: break;
+    }
+    return a;
+}
+
+// Apply an action to a 2^k vector whose local bit j is global qubit qs[j].
+void apply_local(const GateAction &a, const int *qs, int k, cd *v)
+{
+    auto local = [&](int q) {
+        for (int j = 0; j < k; j++)
+            if (qs[j] == q) return j;
+        return -1;
+    };
+    uint64_t cm = 0;
+    for (int i = 0; i < a.nctrl; i++) cm |= 1ull << local(a.ctrl[i]);
+    const int dim = 1 << k;
+    if (a.is_swap) {
+        const uint64_t b1 = 1ull << local(a.target), b2 = 1ull << local(a.target2);
+        for (int i = 0; i < dim; i++)
+            if ((i & cm) == cm && (i & b1) && !(i & b2)) std::swap(v[i], v[(i ^ b1) | b2]);
+        return;
+    }
+    const uint64_t tb = 1ull << local(a.target);
+    for (int i = 0; i < dim; i++) {
+        if ((i & cm) != cm || (i & tb)) continue;
+        const cd x0 = v[i], x1 = v[i | tb];
+        v[i] = a.u[0] * x0 + a.u[1] * x1;
+        v[i | tb] = a.u[2] * x0 + a.u[3] * x1;
+    }
+}
+
+}  // namespace
+
+int gate_qubits(const moonlab_hip_gate_t &g, int q[3])
+{
+    int n = 0;
+    q[n++] = g.target;
+    const bool two = (g.type >= MOONLAB_HIP_GATE_CNOT && g.type <= MOONLAB_HIP_GATE_SWAP) ||
+                     g.type >= MOONLAB_HIP_GATE_CPHASE;
+    if (two) q[n++] = g.control;
+    if (g.type == MOONLAB_HIP_GATE_TOFFOLI || g.type == MOONLAB_HIP_GATE_FREDKIN) q[n++] = g.extra;
+    std::sort(q, q + n);
+    return n;
+}
+
+void gate_dense(const moonlab_hip_gate_t &g, cd *m)
+{
+    int q[3];
+    const int k = gate_qubits(g, q);
+    const GateAction a = action_of(g);
+    const int dim = 1 << k;
+    std::vector<cd> col((size_t)dim);
+    for (int c = 0; c < dim; c++) {
+        std::fill(col.begin(), col.end(), cd(0.0));
+        col[(size_t)c] = 1.0;
+        apply_local(a, q, k, col.data());
+        for (int r = 0; r < dim; r++) m[(size_t)r * dim + c] = col[(size_t)r];
+    }
+}
+
+std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kmax)
+{
+    std::vector<FusedBlock> blocks;
+    std::vector<int> cur;                       // qubits of the open block
+    std::vector<const moonlab_hip_gate_t *> members;
+    auto flush = [&]() {
+        if (members.empty()) return;
+        FusedBlock b;
+        b.qubits = cur;
+        const int k = (int)cur.size(), dim = 1 << k;
+        b.mat.assign((size_t)dim * dim, cd(0.0));
+        for (int i = 0; i < dim; i++) b.mat[(size_t)i * dim + i] = 1.0;
+        std::vector<cd> col((size_t)dim);
+        for (const moonlab_hip_gate_t *gp : members) {
+            if (gp->type == MOONLAB_HIP_GATE_I) continue;
+            const GateAction a = action_of(*gp);
+            for (int c = 0; c < dim; c++) {
+                for (int r = 0; r < dim; r++) col[(size_t)r] = b.mat[(size_t)r * dim + c];
+                apply_local(a, cur.data(), k, col.data());
+                for (int r = 0; r < dim; r++) b.mat[(size_t)r * dim + c] = col[(size_t)r];
+            }
+        }
+        b.num_gates = (int)members.size();
+        blocks.push_back(std::move(b));
+        cur.clear();
+        members.clear();
+    };
+    for (int i = 0; i < ng; i++) {
+        int q[3];
+        const int nq = gate_qubits(g[i], q);
+        std::vector<int> uni = cur;
+        for (int j = 0; j < nq; j++)
+            if (std::find(uni.begin(), uni.end(), q[j]) == uni.end()) uni.push_back(q[j]);
+        std::sort(uni.begin(), uni.end());
+        if ((int)uni.size() > kmax && !members.empty()) {
+            flush();
+            uni.assign(q, q + nq);
+        }
+        cur = uni;
+        members.push_back(&g[i]);
+    }
+    flush();
+    return blocks;
+}
+
+int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int kmax)
+{
+    for (int i = 0; i < ng; i++) {
+        int q[3];
+        const int nq = gate_qubits(g[i], q);
+        for (int j = 0; j < nq; j++)
+            if (q[j] < 0 || q[j] >= (int)moonlab_hip_state_n_qubits(s)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
+        for (int j = 1; j < nq; j++)
+            if (q[j] == q[j - 1]) return MOONLAB_HIP_ERR_INVALID_QUBIT;
+        if (g[i].type < 0 || g[i].type > MOONLAB_HIP_GATE_FREDKIN) return MOONLAB_HIP_ERR_BAD_ARG;
+    }
+    if (kmax < 1) kmax = 1;
+    if (kmax > 5) kmax = 5;
+    const std::vector<FusedBlock> blocks = plan_fusion(g, ng, kmax);
+    std::vector<double> m;
+    for (const FusedBlock &b : blocks) {
+        const int k = (int)b.qubits.size();
+        m.resize(b.mat.size() * 2);
+        for (size_t i = 0; i < b.mat.size(); i++) {
+            m[2 * i] = b.mat[i].real();
+            m[2 * i + 1] = b.mat[i].imag();
+        }
+        const int st = moonlab_hip_apply_dense(s, k, b.qubits.data(), m.data());
+        if (st) return st;
+    }
+    return MOONLAB_HIP_OK;
+}
+
+}  // namespace mlx
+
+extern "C" {
+
+moonlab_hip_status_t moonlab_hip_apply_dense(moonlab_hip_state_t *s, int k, const int *qubits, const double *mat)
+{
+    if (!s || !qubits || !mat || k < 1 || k > 5) return MOONLAB_HIP_ERR_BAD_ARG;
+    const int n = (int)moonlab_hip_state_n_qubits(s);
+    for (int j = 0; j < k; j++) {
+        if (qubits[j] < 0 || qubits[j] >= n) return MOONLAB_HIP_ERR_INVALID_QUBIT;
+        if (j && qubits[j] <= qubits[j - 1]) return MOONLAB_HIP_ERR_BAD_ARG;
+    }
+    const mlx::SvDev d{moonlab_hip_state_device_ptr(s), n, moonlab_hip_state_precision(s),
+                       (hipStream_t)moonlab_hip_state_stream(s)};
+    hipError_t e;
+    if (k == 1) {
+        e = mlx::launch_matrix_1q(d, qubits[0], mat);
+    } else if (k == 2) {
+        e = mlx::launch_matrix_2q(d, qubits[0], qubits[1], mat);
+    } else {
+        e = mlx::launch_dense(d, k, qubits, mat);
+    }
+    if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
+    return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
+}
+
+moonlab_hip_status_t moonlab_hip_apply_gates_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng,
+                                                   int kmax)
+{
+    if (!s || (ng > 0 && !g) || ng < 0 || kmax < 1 || kmax > 5) return MOONLAB_HIP_ERR_BAD_ARG;
+    return (moonlab_hip_status_t)mlx::apply_fused(s, g, ng, kmax);
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// fusion.h -- k-qubit gate fusion (k <= 5): plan, FP64 composition, apply.
+#pragma once
+
+#include <stdint.h>
+
+#include <complex>
+#include <vector>
+
+#include "../../include/moonlab_hip_statevec.h"
+
+namespace mlx {
+
+using cd = std::complex<double>;
+
+// A fused block: the sorted qubits it acts on and its 2^k x 2^k unitary,
+// row-major, local basis bit j = qubits[j].
+struct FusedBlock {
+    std::vector<int> qubits;
+    std::vector<cd>  mat;
+    int              num_gates = 0;
+};
+
+// Qubits a gate acts on (1..3); returns the count.
+int gate_qubits(const moonlab_hip_gate_t &g, int q[3]);
+
+// Dense matrix of a gate on its own qubits (local bit j = q[j] from
+// gate_qubits), row-major 2^nq x 2^nq.
+void gate_dense(const moonlab_hip_gate_t &g, cd *m);
+
+// Greedy in-order fusion into blocks of at most kmax qubits, each composed
+// in FP64 (left-multiplication in gate order).
+std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kmax);
+
+// Apply a gate list through fusion (returns a moonlab_hip_status_t value).
+int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int kmax);
+
+}  // namespace mlx

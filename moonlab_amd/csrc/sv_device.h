@@ -1,0 +1,153 @@
+// sv_device.h -- device-side building blocks shared by the state-vector kernels.
+//
+// Amplitudes are interleaved (re, im): double2 for complex128, float2 for
+// complex64.  Qubit q is bit q of the basis index (qubit 0 = LSB), the
+// reference's convention (src/quantum/state.h:102-131).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlx {
+
+template <typename R> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+template <typename R> using cplx_t = typename Cplx<R>::type;
+
+// Insert a zero bit at position q of p (pair / subspace enumeration).
+__device__ __forceinline__ uint64_t ins0(uint64_t p, int q)
+{
+    const uint64_t lo = p & ((1ull << q) - 1ull);
+    return ((p - lo) << 1) | lo;
+}
+
+// Up to 7 fixed bit positions, ascending.  Depositing a sub-index into the
+// complement of these positions enumerates every basis index whose fixed
+// bits are all zero.
+struct BitSet {
+    int nb;
+    int pos[7];
+};
+
+__device__ __forceinline__ uint64_t deposit(uint64_t p, const BitSet &bs)
+{
+    for (int k = 0; k < bs.nb; k++) p = ins0(p, bs.pos[k]);
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Pair operators: (a0, a1) -> (a0', a1') for the two amplitudes that differ
+// only in the target bit.  Each mirrors the per-pair arithmetic of the
+// reference gate it implements (src/quantum/gates.c lines cited).
+// ---------------------------------------------------------------------------
+
+template <typename R> struct OpSwap {            // X, CNOT, Toffoli (gates.c:123-154, 571-639)
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> t = a; a = b; b = t;
+    }
+};
+
+template <typename R> struct OpY {               // Y = [[0,-i],[i,0]] (gates.c:156-192)
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> a0 = a;
+        a.x = b.y;  a.y = -b.x;
+        b.x = -a0.y; b.y = a0.x;
+    }
+};
+
+template <typename R> struct OpNegY {            // -Y = S^dag X S: the job path's CY body
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {                                             // (moonlab_qgtl_backend.c:138-144)
+        const cplx_t<R> a0 = a;
+        a.x = -b.y; a.y = b.x;
+        b.x = a0.y; b.y = -a0.x;
+    }
+};
+
+template <typename R> struct OpH {               // (a0 +- a1) * INV_SQRT2 (simd_ops.c:1026-1103)
+    R r;
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> a0 = a;
+        a.x = (a0.x + b.x) * r; a.y = (a0.y + b.y) * r;
+        b.x = (a0.x - b.x) * r; b.y = (a0.y - b.y) * r;
+    }
+};
+
+template <typename R> struct OpRX {              // [[c,-is],[-is,c]] (gates.c:385-427)
+    R c, s;
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> a0 = a;
+        a.x = c * a0.x + s * b.y;  a.y = c * a0.y - s * b.x;
+        b.x = c * b.x + s * a0.y;  b.y = c * b.y - s * a0.x;
+    }
+};
+
+template <typename R> struct OpRY {              // [[c,-s],[s,c]] (gates.c:429-479)
+    R c, s;
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> a0 = a;
+        a.x = c * a0.x - s * b.x;  a.y = c * a0.y - s * b.y;
+        b.x = s * a0.x + c * b.x;  b.y = s * a0.y + c * b.y;
+    }
+};
+
+template <typename R> struct OpDiag2 {           // diag(d0, d1) on a pair (controlled RZ)
+    R d0r, d0i, d1r, d1i;
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> a0 = a, b0 = b;
+        a.x = a0.x * d0r - a0.y * d0i; a.y = a0.x * d0i + a0.y * d0r;
+        b.x = b0.x * d1r - b0.y * d1i; b.y = b0.x * d1i + b0.y * d1r;
+    }
+};
+
+template <typename R> struct OpMat2 {            // general 2x2, row-major (re, im) (gates.c:1381-1390)
+    R m[8];
+    __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
+    {
+        const cplx_t<R> a0 = a, b0 = b;
+        a.x = (m[0] * a0.x - m[1] * a0.y) + (m[2] * b0.x - m[3] * b0.y);
+        a.y = (m[0] * a0.y + m[1] * a0.x) + (m[2] * b0.y + m[3] * b0.x);
+        b.x = (m[4] * a0.x - m[5] * a0.y) + (m[6] * b0.x - m[7] * b0.y);
+        b.y = (m[4] * a0.y + m[5] * a0.x) + (m[6] * b0.y + m[7] * b0.x);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Element operators for diagonal gates.
+// ---------------------------------------------------------------------------
+
+template <typename R> struct ElNeg {             // Z, CZ, MCZ (gates.c:194-224, 641-668)
+    __device__ __forceinline__ void operator()(cplx_t<R> &a) const { a.x = -a.x; a.y = -a.y; }
+};
+
+template <typename R> struct ElMulI {            // S: times +i (gates.c:309-339)
+    __device__ __forceinline__ void operator()(cplx_t<R> &a) const
+    {
+        const R r = a.x; a.x = -a.y; a.y = r;
+    }
+};
+
+template <typename R> struct ElMulNegI {         // S-dagger: times -i (gates.c:341-371)
+    __device__ __forceinline__ void operator()(cplx_t<R> &a) const
+    {
+        const R r = a.x; a.x = a.y; a.y = -r;
+    }
+};
+
+template <typename R> struct ElPhase {           // T, P, CPHASE (gates.c:276-307, 737-765)
+    R pr, pi;
+    __device__ __forceinline__ void operator()(cplx_t<R> &a) const
+    {
+        const R r = a.x, i = a.y;
+        a.x = r * pr - i * pi; a.y = r * pi + i * pr;
+    }
+};
+
+}  // namespace mlx

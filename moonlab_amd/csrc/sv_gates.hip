@@ -1,0 +1,555 @@
+// sv_gates.hip -- gfx950 gate kernels for the dense state vector.
+//
+// Every kernel is a streaming pass over HBM: each thread owns E pairs (or
+// elements, or quadruples) spaced one block apart, issues all of its loads
+// before any arithmetic, then stores.  Consecutive lanes take consecutive
+// pair indices, so for any target qubit each wave reads whole contiguous
+// runs: for q >= 2 two runs of >= 64 B per 64-lane wave (the |0> and |1>
+// halves), for q < 2 one run that interleaves both halves.  Only the
+// amplitudes a gate changes are touched: diagonal gates on one half (S, T,
+// Z) read and write D/2 amplitudes, controlled gates enumerate only the
+// control = 1 subspace (D/2^(k+1) pairs for k controls) by depositing the
+// sub-index around the fixed bits, so no lane idles on a failed control test.
+//
+// Reference counterparts: src/backends/cuda_statevec.cu:81-390 (one thread
+// per pair, half the CNOT threads idle, 2q matrix in a __constant__ symbol)
+// and the CPU stride-block loops of src/quantum/gates.c.
+#include "sv_device.h"
+#include "sv_internal.h"
+
+#include <math.h>
+
+namespace mlx {
+
+GateTuning &gate_tuning()
+{
+    static GateTuning t = {4, 256};
+    return t;
+}
+
+namespace {
+
+constexpr int kMaxBlock = 1024;
+
+// --- pair kernels -----------------------------------------------------------
+
+template <typename R, typename Op, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_pair(cplx_t<R> *__restrict__ a, uint64_t npairs,
+                                                    int q, Op op)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t stride = 1ull << q;
+    if (base + (uint64_t)(E - 1) * bs < npairs) {
+        uint64_t i0[E];
+        cplx_t<R> x0[E], x1[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            i0[k] = ins0(base + (uint64_t)k * bs, q);
+            x0[k] = a[i0[k]];
+            x1[k] = a[i0[k] + stride];
+        }
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            op(x0[k], x1[k]);
+            a[i0[k]] = x0[k];
+            a[i0[k] + stride] = x1[k];
+        }
+    } else {
+        for (int k = 0; k < E; k++) {
+            const uint64_t p = base + (uint64_t)k * bs;
+            if (p >= npairs) break;
+            const uint64_t j = ins0(p, q);
+            cplx_t<R> x0 = a[j], x1 = a[j + stride];
+            op(x0, x1);
+            a[j] = x0;
+            a[j + stride] = x1;
+        }
+    }
+}
+
+// Only the |1> half of each pair (phase-on-one gates).
+template <typename R, typename Op, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_half(cplx_t<R> *__restrict__ a, uint64_t nhalf,
+                                                    int q, Op op)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t bit = 1ull << q;
+    if (base + (uint64_t)(E - 1) * bs < nhalf) {
+        uint64_t i[E];
+        cplx_t<R> x[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            i[k] = ins0(base + (uint64_t)k * bs, q) | bit;
+            x[k] = a[i[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            op(x[k]);
+            a[i[k]] = x[k];
+        }
+    } else {
+        for (int k = 0; k < E; k++) {
+            const uint64_t p = base + (uint64_t)k * bs;
+            if (p >= nhalf) break;
+            const uint64_t j = ins0(p, q) | bit;
+            cplx_t<R> x = a[j];
+            op(x);
+            a[j] = x;
+        }
+    }
+}
+
+// diag(d0, d1) on qubit q over every amplitude: fully contiguous stream.
+template <typename R, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_diag(cplx_t<R> *__restrict__ a, uint64_t dim,
+                                                    int q, R d0r, R d0i, R d1r, R d1i)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const bool full = base + (uint64_t)(E - 1) * bs < dim;
+    cplx_t<R> x[E];
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const uint64_t i = base + (uint64_t)k * bs;
+        if (full || i < dim) x[k] = a[i];
+    }
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const uint64_t i = base + (uint64_t)k * bs;
+        if (!(full || i < dim)) continue;
+        const bool one = (i >> q) & 1ull;
+        const R pr = one ? d1r : d0r, pi = one ? d1i : d0i;
+        const R r = x[k].x, m = x[k].y;
+        x[k].x = r * pr - m * pi;
+        x[k].y = r * pi + m * pr;
+        a[i] = x[k];
+    }
+}
+
+// Controlled pair: indices with every bit of `fixed` zero are enumerated,
+// then the control bits are set; the pair is (b, b | tbit).
+template <typename R, typename Op, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_ctrl_pair(cplx_t<R> *__restrict__ a, uint64_t nsub,
+                                                         BitSet fixed, uint64_t cmask,
+                                                         uint64_t tbit, Op op)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    if (base + (uint64_t)(E - 1) * bs < nsub) {
+        uint64_t i0[E];
+        cplx_t<R> x0[E], x1[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            i0[k] = deposit(base + (uint64_t)k * bs, fixed) | cmask;
+            x0[k] = a[i0[k]];
+            x1[k] = a[i0[k] | tbit];
+        }
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            op(x0[k], x1[k]);
+            a[i0[k]] = x0[k];
+            a[i0[k] | tbit] = x1[k];
+        }
+    } else {
+        for (int k = 0; k < E; k++) {
+            const uint64_t p = base + (uint64_t)k * bs;
+            if (p >= nsub) break;
+            const uint64_t j = deposit(p, fixed) | cmask;
+            cplx_t<R> x0 = a[j], x1 = a[j | tbit];
+            op(x0, x1);
+            a[j] = x0;
+            a[j | tbit] = x1;
+        }
+    }
+}
+
+// Element op where every bit of `mask` is 1 (CZ, CPHASE, MCZ).
+template <typename R, typename Op, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_mask_elem(cplx_t<R> *__restrict__ a, uint64_t nsub,
+                                                         BitSet fixed, uint64_t mask, Op op)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    if (base + (uint64_t)(E - 1) * bs < nsub) {
+        uint64_t i[E];
+        cplx_t<R> x[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            i[k] = deposit(base + (uint64_t)k * bs, fixed) | mask;
+            x[k] = a[i[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            op(x[k]);
+            a[i[k]] = x[k];
+        }
+    } else {
+        for (int k = 0; k < E; k++) {
+            const uint64_t p = base + (uint64_t)k * bs;
+            if (p >= nsub) break;
+            const uint64_t j = deposit(p, fixed) | mask;
+            cplx_t<R> x = a[j];
+            op(x);
+            a[j] = x;
+        }
+    }
+}
+
+// Swap (base|cmask|b1) <-> (base|cmask|b2): SWAP and Fredkin.
+template <typename R, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_swap_bits(cplx_t<R> *__restrict__ a, uint64_t nsub,
+                                                         BitSet fixed, uint64_t cmask,
+                                                         uint64_t b1, uint64_t b2)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const uint64_t p = base + (uint64_t)k * bs;
+        if (p >= nsub) break;
+        const uint64_t j = deposit(p, fixed) | cmask;
+        const cplx_t<R> x0 = a[j | b1], x1 = a[j | b2];
+        a[j | b1] = x1;
+        a[j | b2] = x0;
+    }
+}
+
+template <typename R> struct Mat4 {
+    R m[32];
+};
+
+// 4x4 complex matvec on the quadruple (b, b|b0, b|b1, b|b0|b1), basis |q1 q0>.
+template <typename R, int E>
+__global__ __launch_bounds__(kMaxBlock) void k_mat4(cplx_t<R> *__restrict__ a, uint64_t nquad,
+                                                    BitSet fixed, uint64_t b0, uint64_t b1,
+                                                    Mat4<R> M)
+{
+    const uint64_t bs = blockDim.x;
+    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const uint64_t p = base + (uint64_t)k * bs;
+        if (p >= nquad) break;
+        const uint64_t j = deposit(p, fixed);
+        const uint64_t idx[4] = {j, j | b0, j | b1, j | b0 | b1};
+        cplx_t<R> v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[c] = a[idx[c]];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            R sr = 0, si = 0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const R mr = M.m[r * 8 + c * 2], mi = M.m[r * 8 + c * 2 + 1];
+                sr += mr * v[c].x - mi * v[c].y;
+                si += mr * v[c].y + mi * v[c].x;
+            }
+            a[idx[r]] = cplx_t<R>{sr, si};
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_init_zero(cplx_t<R> *__restrict__ a, uint64_t dim)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride)
+        a[i] = cplx_t<R>{i == 0 ? R(1) : R(0), R(0)};
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scale(cplx_t<R> *__restrict__ a, uint64_t dim, R f)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        cplx_t<R> x = a[i];
+        x.x *= f;
+        x.y *= f;
+        a[i] = x;
+    }
+}
+
+template <typename RS, typename RD>
+__global__ __launch_bounds__(256) void k_convert(const cplx_t<RS> *__restrict__ s,
+                                                 cplx_t<RD> *__restrict__ d, uint64_t count)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+        d[i] = cplx_t<RD>{(RD)s[i].x, (RD)s[i].y};
+}
+
+// --- launch helpers -----------------------------------------------------------
+
+inline unsigned grid_for(uint64_t work, int block, int e)
+{
+    const uint64_t per = (uint64_t)block * (uint64_t)e;
+    return (unsigned)((work + per - 1) / per);
+}
+
+inline unsigned grid_stride_blocks(uint64_t work)
+{
+    const uint64_t b = (work + 255) / 256;
+    return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+#define MLX_DISPATCH_E(EV, ...)                 \
+    switch (EV) {                               \
+    case 1: { constexpr int E = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int E = 2; __VA_ARGS__; } break; \
+    case 8: { constexpr int E = 8; __VA_ARGS__; } break; \
+    default: { constexpr int E = 4; __VA_ARGS__; } break; \
+    }
+
+template <typename R, typename Op>
+hipError_t run_pair(const SvDev &s, int q, Op op)
+{
+    const uint64_t np = (1ull << s.n) >> 1;
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_pair<R, Op, E><<<grid_for(np, t.block, E), t.block, 0, s.stream>>>(a, np, q, op));
+    return hipGetLastError();
+}
+
+template <typename R, typename Op>
+hipError_t run_half(const SvDev &s, int q, Op op)
+{
+    const uint64_t nh = (1ull << s.n) >> 1;
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_half<R, Op, E><<<grid_for(nh, t.block, E), t.block, 0, s.stream>>>(a, nh, q, op));
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t run_diag(const SvDev &s, int q, const double d[4])
+{
+    const uint64_t dim = 1ull << s.n;
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_diag<R, E><<<grid_for(dim, t.block, E), t.block, 0, s.stream>>>(
+                              a, dim, q, (R)d[0], (R)d[1], (R)d[2], (R)d[3]));
+    return hipGetLastError();
+}
+
+inline BitSet make_bitset(uint64_t bits)
+{
+    BitSet b;
+    b.nb = 0;
+    for (int i = 0; i < 64 && b.nb < 7; i++)
+        if (bits & (1ull << i)) b.pos[b.nb++] = i;
+    return b;
+}
+
+template <typename R, typename Op>
+hipError_t run_ctrl_pair(const SvDev &s, uint64_t cmask, int target, Op op)
+{
+    const uint64_t tbit = 1ull << target;
+    const BitSet fixed = make_bitset(cmask | tbit);
+    const uint64_t nsub = (1ull << s.n) >> fixed.nb;
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_ctrl_pair<R, Op, E><<<grid_for(nsub, t.block, E), t.block, 0, s.stream>>>(
+                              a, nsub, fixed, cmask, tbit, op));
+    return hipGetLastError();
+}
+
+template <typename R, typename Op>
+hipError_t run_mask_elem(const SvDev &s, uint64_t mask, Op op)
+{
+    const BitSet fixed = make_bitset(mask);
+    const uint64_t nsub = (1ull << s.n) >> fixed.nb;
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_mask_elem<R, Op, E><<<grid_for(nsub, t.block, E), t.block, 0, s.stream>>>(
+                              a, nsub, fixed, mask, op));
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t run_swap(const SvDev &s, uint64_t cmask, int q1, int q2)
+{
+    const uint64_t b1 = 1ull << q1, b2 = 1ull << q2;
+    const BitSet fixed = make_bitset(cmask | b1 | b2);
+    const uint64_t nsub = (1ull << s.n) >> fixed.nb;
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_swap_bits<R, E><<<grid_for(nsub, t.block, E), t.block, 0, s.stream>>>(
+                              a, nsub, fixed, cmask, b1, b2));
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t run_mat4(const SvDev &s, int q0, int q1, const double m[32])
+{
+    const uint64_t b0 = 1ull << q0, b1 = 1ull << q1;
+    const BitSet fixed = make_bitset(b0 | b1);
+    const uint64_t nq = (1ull << s.n) >> 2;
+    Mat4<R> M;
+    for (int i = 0; i < 32; i++) M.m[i] = (R)m[i];
+    const GateTuning t = gate_tuning();
+    auto *a = (cplx_t<R> *)s.amps;
+    MLX_DISPATCH_E(t.ept, k_mat4<R, E><<<grid_for(nq, t.block, E), t.block, 0, s.stream>>>(
+                              a, nq, fixed, b0, b1, M));
+    return hipGetLastError();
+}
+
+template <typename R> OpMat2<R> mat2(const double m[8])
+{
+    OpMat2<R> o;
+    for (int i = 0; i < 8; i++) o.m[i] = (R)m[i];
+    return o;
+}
+
+// One gate of the code table (include/moonlab_hip_statevec.h) at precision R.
+template <typename R>
+hipError_t gate_r(const SvDev &s, const moonlab_hip_gate_t &g)
+{
+    const int t = g.target, c = g.control, e = g.extra;
+    switch (g.type) {
+    case MOONLAB_HIP_GATE_I: return hipSuccess;
+    case MOONLAB_HIP_GATE_X: return run_pair<R>(s, t, OpSwap<R>{});
+    case MOONLAB_HIP_GATE_Y: return run_pair<R>(s, t, OpY<R>{});
+    case MOONLAB_HIP_GATE_Z: return run_half<R>(s, t, ElNeg<R>{});
+    case MOONLAB_HIP_GATE_H: return run_pair<R>(s, t, OpH<R>{(R)0.70710678118654752440});
+    case MOONLAB_HIP_GATE_S: return run_half<R>(s, t, ElMulI<R>{});
+    case MOONLAB_HIP_GATE_SDG: return run_half<R>(s, t, ElMulNegI<R>{});
+    case MOONLAB_HIP_GATE_T:
+        return run_half<R>(s, t, ElPhase<R>{(R)cos(M_PI / 4.0), (R)sin(M_PI / 4.0)});
+    case MOONLAB_HIP_GATE_TDG:
+        return run_half<R>(s, t, ElPhase<R>{(R)cos(-M_PI / 4.0), (R)sin(-M_PI / 4.0)});
+    case MOONLAB_HIP_GATE_PHASE:
+        return run_half<R>(s, t, ElPhase<R>{(R)cos(g.theta), (R)sin(g.theta)});
+    case MOONLAB_HIP_GATE_RX:
+        return run_pair<R>(s, t, OpRX<R>{(R)cos(g.theta / 2.0), (R)sin(g.theta / 2.0)});
+    case MOONLAB_HIP_GATE_RY:
+        return run_pair<R>(s, t, OpRY<R>{(R)cos(g.theta / 2.0), (R)sin(g.theta / 2.0)});
+    case MOONLAB_HIP_GATE_RZ: {
+        const double co = cos(g.theta / 2.0), si = sin(g.theta / 2.0);
+        const double d[4] = {co, -si, co, si};
+        return run_diag<R>(s, t, d);
+    }
+    case MOONLAB_HIP_GATE_U3: {
+        const double co = cos(g.theta / 2.0), si = sin(g.theta / 2.0);
+        const double m[8] = {co, 0.0, -cos(g.lam) * si, -sin(g.lam) * si,
+                             cos(g.phi) * si, sin(g.phi) * si,
+                             cos(g.phi + g.lam) * co, sin(g.phi + g.lam) * co};
+        return run_pair<R>(s, t, mat2<R>(m));
+    }
+    case MOONLAB_HIP_GATE_CNOT: return run_ctrl_pair<R>(s, 1ull << c, t, OpSwap<R>{});
+    case MOONLAB_HIP_GATE_CY: return run_ctrl_pair<R>(s, 1ull << c, t, OpNegY<R>{});
+    case MOONLAB_HIP_GATE_CY_TRUE: return run_ctrl_pair<R>(s, 1ull << c, t, OpY<R>{});
+    case MOONLAB_HIP_GATE_CZ: return run_mask_elem<R>(s, (1ull << c) | (1ull << t), ElNeg<R>{});
+    case MOONLAB_HIP_GATE_SWAP: return run_swap<R>(s, 0ull, c, t);
+    case MOONLAB_HIP_GATE_CPHASE:
+        return run_mask_elem<R>(s, (1ull << c) | (1ull << t),
+                                ElPhase<R>{(R)cos(g.theta), (R)sin(g.theta)});
+    case MOONLAB_HIP_GATE_CRX:
+        return run_ctrl_pair<R>(s, 1ull << c, t,
+                                OpRX<R>{(R)cos(g.theta / 2.0), (R)sin(g.theta / 2.0)});
+    case MOONLAB_HIP_GATE_CRY:
+        return run_ctrl_pair<R>(s, 1ull << c, t,
+                                OpRY<R>{(R)cos(g.theta / 2.0), (R)sin(g.theta / 2.0)});
+    case MOONLAB_HIP_GATE_CRZ: {
+        const R co = (R)cos(g.theta / 2.0), si = (R)sin(g.theta / 2.0);
+        return run_ctrl_pair<R>(s, 1ull << c, t, OpDiag2<R>{co, -si, co, si});
+    }
+    case MOONLAB_HIP_GATE_TOFFOLI:
+        return run_ctrl_pair<R>(s, (1ull << c) | (1ull << e), t, OpSwap<R>{});
+    case MOONLAB_HIP_GATE_FREDKIN: return run_swap<R>(s, 1ull << c, t, e);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_gate(const SvDev &s, const moonlab_hip_gate_t &g)
+{
+    return s.prec == MOONLAB_HIP_C64 ? gate_r<float>(s, g) : gate_r<double>(s, g);
+}
+
+hipError_t launch_matrix_1q(const SvDev &s, int q, const double m[8])
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_pair<float>(s, q, mat2<float>(m))
+                                     : run_pair<double>(s, q, mat2<double>(m));
+}
+
+hipError_t launch_matrix_2q(const SvDev &s, int q0, int q1, const double m[32])
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_mat4<float>(s, q0, q1, m)
+                                     : run_mat4<double>(s, q0, q1, m);
+}
+
+hipError_t launch_controlled_1q(const SvDev &s, uint64_t cmask, int target, const double m[8])
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_ctrl_pair<float>(s, cmask, target, mat2<float>(m))
+                                     : run_ctrl_pair<double>(s, cmask, target, mat2<double>(m));
+}
+
+hipError_t launch_mcx(const SvDev &s, uint64_t cmask, int target)
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_ctrl_pair<float>(s, cmask, target, OpSwap<float>{})
+                                     : run_ctrl_pair<double>(s, cmask, target, OpSwap<double>{});
+}
+
+hipError_t launch_mcz(const SvDev &s, uint64_t mask)
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_mask_elem<float>(s, mask, ElNeg<float>{})
+                                     : run_mask_elem<double>(s, mask, ElNeg<double>{});
+}
+
+hipError_t launch_phase_mask(const SvDev &s, uint64_t mask, double pr, double pi)
+{
+    return s.prec == MOONLAB_HIP_C64
+               ? run_mask_elem<float>(s, mask, ElPhase<float>{(float)pr, (float)pi})
+               : run_mask_elem<double>(s, mask, ElPhase<double>{pr, pi});
+}
+
+hipError_t launch_cswap(const SvDev &s, uint64_t cmask, int t1, int t2)
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_swap<float>(s, cmask, t1, t2)
+                                     : run_swap<double>(s, cmask, t1, t2);
+}
+
+hipError_t launch_diag_1q(const SvDev &s, int q, const double d[4])
+{
+    return s.prec == MOONLAB_HIP_C64 ? run_diag<float>(s, q, d) : run_diag<double>(s, q, d);
+}
+
+hipError_t launch_init_zero(const SvDev &s)
+{
+    const uint64_t dim = 1ull << s.n;
+    if (s.prec == MOONLAB_HIP_C64)
+        k_init_zero<float><<<grid_stride_blocks(dim), 256, 0, s.stream>>>((float2 *)s.amps, dim);
+    else
+        k_init_zero<double><<<grid_stride_blocks(dim), 256, 0, s.stream>>>((double2 *)s.amps, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(const SvDev &s, double f)
+{
+    const uint64_t dim = 1ull << s.n;
+    if (s.prec == MOONLAB_HIP_C64)
+        k_scale<float><<<grid_stride_blocks(dim), 256, 0, s.stream>>>((float2 *)s.amps, dim, (float)f);
+    else
+        k_scale<double><<<grid_stride_blocks(dim), 256, 0, s.stream>>>((double2 *)s.amps, dim, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const void *src, int sp, void *dst, int dp, uint64_t count,
+                          hipStream_t st)
+{
+    const unsigned g = grid_stride_blocks(count);
+    if (sp == MOONLAB_HIP_C128 && dp == MOONLAB_HIP_C64)
+        k_convert<double, float><<<g, 256, 0, st>>>((const double2 *)src, (float2 *)dst, count);
+    else if (sp == MOONLAB_HIP_C64 && dp == MOONLAB_HIP_C128)
+        k_convert<float, double><<<g, 256, 0, st>>>((const float2 *)src, (double2 *)dst, count);
+    else if (sp == MOONLAB_HIP_C128)
+        k_convert<double, double><<<g, 256, 0, st>>>((const double2 *)src, (double2 *)dst, count);
+    else
+        k_convert<float, float><<<g, 256, 0, st>>>((const float2 *)src, (float2 *)dst, count);
+    return hipGetLastError();
+}
+
+}  // namespace mlx

@@ -1,0 +1,247 @@
+"""ctypes access to the two CPU checkers.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` -- never by the product package
+``moonlab_amd`` (which refuses to run without its HIP library).
+
+* :class:`Oracle` wraps ``oracle/libsv_oracle.so``, this repository's C
+  restatement of the reference's dense state-vector semantics.
+* :class:`RefLib` wraps ``oracle/_ref/libmoonlab_ref.so``, the reference's own
+  C compiled from its sources by ``oracle/Makefile`` (``make -C oracle ref``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libsv_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmoonlab_ref.so")
+
+# Gate codes shared with oracle/sv_oracle.h (0..13 = moonlab_qgtl_gate_t).
+I, X, Y, Z, H, S, T, RX, RY, RZ, CNOT, CY, CZ, SWAP = range(14)
+SDG, TDG, PHASE, U3, CPHASE, CRX, CRY, CRZ, CY_TRUE, TOFFOLI, FREDKIN = range(14, 25)
+
+GATE_DTYPE = np.dtype([("type", "<i4"), ("target", "<i4"), ("control", "<i4"),
+                       ("extra", "<i4"), ("theta", "<f8"), ("phi", "<f8"),
+                       ("lam", "<f8")])
+
+
+def gates_array(gates: Iterable[Sequence]) -> np.ndarray:
+    """Pack (type, target, control, extra, theta, phi, lam) tuples; missing
+    trailing fields default to -1 / 0.0."""
+    rows = []
+    for g in gates:
+        g = tuple(g)
+        t = g[0]
+        tgt = g[1]
+        ctl = g[2] if len(g) > 2 else -1
+        ext = g[3] if len(g) > 3 else -1
+        th = g[4] if len(g) > 4 else 0.0
+        ph = g[5] if len(g) > 5 else 0.0
+        la = g[6] if len(g) > 6 else 0.0
+        rows.append((t, tgt, ctl, ext, th, ph, la))
+    return np.array(rows, dtype=GATE_DTYPE)
+
+
+def _ptr(a: np.ndarray, ct=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class Oracle:
+    """The C restatement (oracle/sv_oracle.c)."""
+
+    def __init__(self, path: str = ORACLE_SO) -> None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.svo_init_zero.argtypes = [ctypes.c_int, dp]
+        L.svo_apply_gates.argtypes = [ctypes.c_int, dp, vp, ctypes.c_int]
+        L.svo_apply_gates.restype = ctypes.c_int
+        L.svo_apply_matrix_1q.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp]
+        L.svo_apply_matrix_2q.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp]
+        L.svo_probabilities.argtypes = [ctypes.c_int, dp, dp]
+        L.svo_probability_one.argtypes = [ctypes.c_int, dp, ctypes.c_int]
+        L.svo_probability_one.restype = ctypes.c_double
+        L.svo_expect_z_all.argtypes = [ctypes.c_int, dp, dp]
+        for f in ("svo_expect_x", "svo_expect_y"):
+            getattr(L, f).argtypes = [ctypes.c_int, dp, ctypes.c_int]
+            getattr(L, f).restype = ctypes.c_double
+        L.svo_correlation_zz.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
+        L.svo_correlation_zz.restype = ctypes.c_double
+        L.svo_norm_sq.argtypes = [ctypes.c_int, dp]
+        L.svo_norm_sq.restype = ctypes.c_double
+        L.svo_fidelity.argtypes = [ctypes.c_int, dp, dp]
+        L.svo_fidelity.restype = ctypes.c_double
+        L.svo_measure_all_outcome.argtypes = [ctypes.c_int, dp, ctypes.c_double]
+        L.svo_measure_all_outcome.restype = ctypes.c_uint64
+        L.svo_splitmix64.argtypes = [ctypes.c_uint64]
+        L.svo_splitmix64.restype = ctypes.c_uint64
+        L.svo_qgtl_shots.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_uint64, u64p]
+        L.svo_job_shots.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_uint64, u64p]
+        L.svo_job_shots.restype = ctypes.c_int
+        L.svo_hea_tfim_energy.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, dp]
+        L.svo_hea_tfim_energy.restype = ctypes.c_double
+        L.svo_compose_unitary.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp,
+                                          ctypes.c_int, dp]
+        L.svo_compose_unitary.restype = ctypes.c_int
+        self.L = L
+
+    # -- state evolution -------------------------------------------------
+    def zero_state(self, n: int) -> np.ndarray:
+        a = np.zeros(1 << n, dtype=np.complex128)
+        a[0] = 1.0
+        return a
+
+    def apply(self, n: int, amps: np.ndarray, gates) -> np.ndarray:
+        g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
+        assert amps.dtype == np.complex128 and amps.flags.c_contiguous
+        if len(g):
+            rc = self.L.svo_apply_gates(n, _ptr(amps.view(np.float64)), g.ctypes.data, len(g))
+            if rc:
+                raise ValueError("oracle rejected a gate")
+        return amps
+
+    def run(self, n: int, gates) -> np.ndarray:
+        return self.apply(n, self.zero_state(n), gates)
+
+    def apply_matrix_1q(self, n, amps, q, m):
+        m = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(2, 2))
+        self.L.svo_apply_matrix_1q(n, _ptr(amps.view(np.float64)), q, _ptr(m.view(np.float64)))
+        return amps
+
+    def apply_matrix_2q(self, n, amps, q1, q2, m):
+        m = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(4, 4))
+        self.L.svo_apply_matrix_2q(n, _ptr(amps.view(np.float64)), q1, q2,
+                                   _ptr(m.view(np.float64)))
+        return amps
+
+    # -- reductions --------------------------------------------------------
+    def probabilities(self, n, amps):
+        p = np.empty(1 << n, dtype=np.float64)
+        self.L.svo_probabilities(n, _ptr(amps.view(np.float64)), _ptr(p))
+        return p
+
+    def expect_z_all(self, n, amps):
+        z = np.empty(n, dtype=np.float64)
+        self.L.svo_expect_z_all(n, _ptr(amps.view(np.float64)), _ptr(z))
+        return z
+
+    def expect_x(self, n, amps, q):
+        return self.L.svo_expect_x(n, _ptr(amps.view(np.float64)), q)
+
+    def expect_y(self, n, amps, q):
+        return self.L.svo_expect_y(n, _ptr(amps.view(np.float64)), q)
+
+    def correlation_zz(self, n, amps, q1, q2):
+        return self.L.svo_correlation_zz(n, _ptr(amps.view(np.float64)), q1, q2)
+
+    def norm_sq(self, n, amps):
+        return self.L.svo_norm_sq(n, _ptr(amps.view(np.float64)))
+
+    def fidelity(self, n, a, b):
+        return self.L.svo_fidelity(n, _ptr(a.view(np.float64)), _ptr(b.view(np.float64)))
+
+    def measure_all_outcome(self, n, amps, u):
+        return int(self.L.svo_measure_all_outcome(n, _ptr(amps.view(np.float64)), u))
+
+    def splitmix64(self, x):
+        return int(self.L.svo_splitmix64(x))
+
+    def qgtl_shots(self, n, amps, shots, seed):
+        out = np.zeros(shots, dtype=np.uint64)
+        self.L.svo_qgtl_shots(n, _ptr(amps.view(np.float64)), shots, seed,
+                              _ptr(out, ctypes.c_uint64))
+        return out
+
+    def job_shots(self, n, amps, shots, workers, seed):
+        out = np.zeros(shots, dtype=np.uint64)
+        w = self.L.svo_job_shots(n, _ptr(amps.view(np.float64)), shots, workers, seed,
+                                 _ptr(out, ctypes.c_uint64))
+        if w < 0:
+            raise ValueError("bad job-shot arguments")
+        return out
+
+    def hea_tfim_energy(self, n, layers, angles, h):
+        angles = np.ascontiguousarray(angles, dtype=np.float64)
+        assert angles.size == layers * n * 2
+        scratch = np.empty(1 << n, dtype=np.complex128)
+        return self.L.svo_hea_tfim_energy(n, layers, _ptr(angles), h,
+                                          _ptr(scratch.view(np.float64)))
+
+    def compose_unitary(self, qubits, gates):
+        g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
+        k = len(qubits)
+        q = (ctypes.c_int * k)(*qubits)
+        mat = np.zeros((1 << k, 1 << k), dtype=np.complex128)
+        rc = self.L.svo_compose_unitary(k, q, g.ctypes.data if len(g) else None, len(g),
+                                        _ptr(mat.view(np.float64)))
+        if rc:
+            raise ValueError("compose_unitary failed")
+        return mat
+
+
+class RefLib:
+    """The reference's own C (oracle/_ref/libmoonlab_ref.so)."""
+
+    def __init__(self, path: str = REF_SO) -> None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref`")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.ref_state_new.argtypes = [ctypes.c_int]
+        L.ref_state_new.restype = vp
+        L.ref_state_delete.argtypes = [vp]
+        L.ref_state_amplitudes.argtypes = [vp]
+        L.ref_state_amplitudes.restype = dp
+        L.ref_apply_gate.argtypes = [vp, vp]
+        L.ref_apply_gate.restype = ctypes.c_int
+        L.ref_apply_gates.argtypes = [vp, vp, ctypes.c_int]
+        L.ref_apply_gates.restype = ctypes.c_int
+        L.ref_run.argtypes = [ctypes.c_int, vp, ctypes.c_int, dp, dp, dp]
+        L.ref_run.restype = ctypes.c_int
+        for f in ("measurement_expectation_x", "measurement_expectation_y",
+                  "measurement_expectation_z", "measurement_probability_one"):
+            getattr(L, f).argtypes = [vp, ctypes.c_int]
+            getattr(L, f).restype = ctypes.c_double
+        L.measurement_correlation_zz.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.measurement_correlation_zz.restype = ctypes.c_double
+        L.measurement_all_qubits.argtypes = [vp, ctypes.c_double]
+        L.measurement_all_qubits.restype = ctypes.c_uint64
+        L.quantum_state_clone.argtypes = [vp, vp]
+        L.quantum_state_clone.restype = ctypes.c_int
+        self.L = L
+
+    def run(self, n: int, gates, want_probs: bool = False):
+        g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
+        amps = np.zeros(1 << n, dtype=np.complex128)
+        z = np.zeros(n, dtype=np.float64)
+        p = np.zeros(1 << n, dtype=np.float64) if want_probs else None
+        rc = self.L.ref_run(n, g.ctypes.data if len(g) else None, len(g),
+                            _ptr(amps.view(np.float64)), _ptr(z),
+                            _ptr(p) if p is not None else None)
+        if rc:
+            raise ValueError(f"reference rejected the circuit (rc={rc})")
+        return (amps, z, p) if want_probs else (amps, z)
+
+    def state(self, n: int, gates):
+        """A live reference quantum_state_t* after the gates (caller frees)."""
+        g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
+        s = self.L.ref_state_new(n)
+        if not s:
+            raise MemoryError("ref_state_new failed")
+        if len(g) and self.L.ref_apply_gates(s, g.ctypes.data, len(g)):
+            self.L.ref_state_delete(s)
+            raise ValueError("reference rejected the circuit")
+        return s
+
+    def free(self, s):
+        self.L.ref_state_delete(s)

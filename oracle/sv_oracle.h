@@ -1,0 +1,108 @@
+/*
+ * sv_oracle.h -- CPU restatement of Moonlab's dense state-vector semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * It is never linked into, called by, or used as a fallback for the product
+ * library (moonlab_amd/libmoonlab_mi355x.so).
+ *
+ * State layout matches the reference: 2^n interleaved (re, im) doubles,
+ * qubit q = bit q of the basis index (src/quantum/state.h:102-131).
+ */
+#ifndef SV_ORACLE_H
+#define SV_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Gate codes.  0..13 equal moonlab_qgtl_gate_t
+ * (src/applications/moonlab_qgtl_backend.h:57-72) with the JOB-PATH meaning
+ * of each tag (CY = S; CNOT; S-dagger, moonlab_qgtl_backend.c:138-144).
+ * 14.. are the wider gate_* family of src/quantum/gates.h. */
+enum {
+    SVO_I = 0, SVO_X = 1, SVO_Y = 2, SVO_Z = 3, SVO_H = 4, SVO_S = 5, SVO_T = 6,
+    SVO_RX = 7, SVO_RY = 8, SVO_RZ = 9, SVO_CNOT = 10, SVO_CY = 11, SVO_CZ = 12,
+    SVO_SWAP = 13,
+    SVO_SDG = 14, SVO_TDG = 15, SVO_PHASE = 16, SVO_U3 = 17, SVO_CPHASE = 18,
+    SVO_CRX = 19, SVO_CRY = 20, SVO_CRZ = 21, SVO_CY_TRUE = 22,
+    SVO_TOFFOLI = 23, SVO_FREDKIN = 24
+};
+
+/* One gate.  `control` is the control (or SWAP's qubit1); `extra` is the
+ * second control of TOFFOLI or the second target of FREDKIN. */
+typedef struct {
+    int32_t type;
+    int32_t target;
+    int32_t control;
+    int32_t extra;
+    double  theta;
+    double  phi;
+    double  lam;
+} svo_gate_t;
+
+/* Initialise |0...0> into amps (2 * 2^n doubles). */
+void svo_init_zero(int n, double *amps);
+
+/* Apply one gate in place.  Returns 0, or -1 on an invalid qubit/type. */
+int svo_apply_gate(int n, double *amps, const svo_gate_t *g);
+
+/* Apply a gate list in place; returns the first error or 0. */
+int svo_apply_gates(int n, double *amps, const svo_gate_t *g, int num_gates);
+
+/* Generic matrices (src/quantum/gates.c:1354-1486).  m2 = 2x2 row-major
+ * (re, im) pairs; m4 = 4x4 row-major in the basis |q2 q1> with qubit1 the
+ * LOW bit (CPU convention, gates.c:1417-1456). */
+void svo_apply_matrix_1q(int n, double *amps, int q, const double m2[8]);
+void svo_apply_matrix_2q(int n, double *amps, int q1, int q2, const double m4[32]);
+
+/* Reductions (src/quantum/measurement.c). */
+void   svo_probabilities(int n, const double *amps, double *p_out);   /* cabs^2 */
+double svo_probability_one(int n, const double *amps, int q);
+void   svo_expect_z_all(int n, const double *amps, double *z_out);
+double svo_expect_x(int n, const double *amps, int q);
+double svo_expect_y(int n, const double *amps, int q);
+double svo_correlation_zz(int n, const double *amps, int q1, int q2);
+double svo_norm_sq(int n, const double *amps);
+double svo_fidelity(int n, const double *a, const double *b);
+
+/* Collapse-free "measure all" outcome for a uniform draw u
+ * (measurement.c:263-307 outcome rule, without the collapse write). */
+uint64_t svo_measure_all_outcome(int n, const double *amps, double u);
+
+/* PRNGs of the job path. */
+uint64_t svo_splitmix64(uint64_t x);
+uint64_t svo_xorshift64(uint64_t *s);
+double   svo_u_from_xorshift(uint64_t *s);
+
+/* moonlab_qgtl_execute's shot loop for one seed (moonlab_qgtl_backend.c:218-243). */
+void svo_qgtl_shots(int n, const double *amps, int num_shots, uint64_t seed,
+                    uint64_t *out);
+
+/* moonlab_scheduler_run's "simulator" fan-out (scheduler.c:194-239):
+ * worker slices and per-worker seeds, outcomes in slice order.  seed != 0. */
+int svo_job_shots(int n, const double *amps, int total_shots, int num_workers,
+                  uint64_t seed, uint64_t *out);
+
+/* Hardware-efficient ansatz (RY, RZ per qubit then CZ(q, q+1) ladder, per
+ * layer) from |0..0>, then the open-chain TFIM energy
+ *   E = -sum_i <Z_i Z_{i+1}> - h sum_i <X_i>
+ * using measurement_correlation_zz / measurement_expectation_x semantics.
+ * angles: layers * n * 2 doubles, laid out [layer][qubit][ry, rz]. */
+double svo_hea_tfim_energy(int n, int layers, const double *angles, double h,
+                           double *amps_scratch);
+
+/* FP64 composition of a list of gates acting only on `qubits` (k <= 5,
+ * ascending) into a 2^k x 2^k row-major complex matrix in the local basis
+ * (local bit j = qubits[j]).  Brute force: applies the gates column by
+ * column to basis vectors of a k-qubit register.  Returns 0 or -1. */
+int svo_compose_unitary(int k, const int *qubits, const svo_gate_t *g,
+                        int num_gates, double *mat_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

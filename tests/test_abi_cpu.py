@@ -1,0 +1,279 @@
+"""The C-ABI boundary without a GPU: symbols, registry, hooks, JSON and the
+circuit text format, mirroring tests/unit/test_scheduler.c and
+tests/unit/test_qgtl_backend.c of the reference.  Nothing here executes a
+circuit on a device."""
+import ctypes
+import json
+import os
+import re
+import threading
+
+import numpy as np
+import pytest
+
+import moonlab_amd as m
+from conftest import GOLDEN, ROOT
+
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared_symbols():
+    names = set()
+    for f in os.listdir(INCLUDE):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for mm in re.finditer(r"MOONLAB_API\s+[^;{]*?\b(moonlab_\w+)\s*\(", text):
+            names.add(mm.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = m.lib()
+    names = declared_symbols()
+    assert len(names) > 80
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    # and nothing exported under the moonlab_ prefix is left undeclared
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", m.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.split()[-1].startswith("moonlab_")}
+    assert exported == names, sorted(exported ^ names)
+
+
+def test_job_introspection_and_errors():
+    L = m.lib()
+    assert not L.moonlab_job_create(0)
+    assert not L.moonlab_job_create(33)
+    j = m.Job(4)
+    assert (j.num_qubits, j.num_gates, j.num_shots, j.num_workers) == (4, 0, 0, 1)
+    j.add_gate(m.GateType.H, 0)
+    assert j.num_gates == 1
+    j.set_num_shots(100).set_num_workers(4)
+    assert (j.num_shots, j.num_workers) == (100, 4)
+    assert L.moonlab_job_set_num_shots(j.handle, -1) == m.MOONLAB_SCHED_BAD_ARG
+    assert L.moonlab_job_set_num_workers(j.handle, 0) == m.MOONLAB_SCHED_BAD_ARG
+    assert L.moonlab_job_add_gate(j.handle, int(m.GateType.RX), 0, -1, None) == m.MOONLAB_SCHED_BAD_ARG
+    assert L.moonlab_job_add_gate(None, 4, 0, -1, None) == m.MOONLAB_SCHED_BAD_ARG
+    # no validation of qubit indices at add time (scheduler.c:99-121)
+    assert L.moonlab_job_add_gate(j.handle, 4, 99, -1, None) == 0
+    j0 = m.Job(2)
+    rc, _ = j0.run_raw()
+    assert rc == m.MOONLAB_SCHED_BAD_ARG  # zero shots
+    assert L.moonlab_scheduler_run_mpi(j0.handle, None, None) == m.MOONLAB_SCHED_NOT_BUILT
+
+
+def test_job_json_byte_identical_to_reference():
+    shots = json.load(open(os.path.join(GOLDEN, "shots_ref.json")))
+    rnd10 = np.load(os.path.join(GOLDEN, "rnd10_gates.npy"))
+    circuits = {"bell": [(4, 0, -1, 0.0), (10, 1, 0, 0.0)],
+                "ghz": [(4, 0, -1, 0.0), (10, 1, 0, 0.0), (10, 2, 1, 0.0)],
+                "rnd10": [(int(g["type"]), int(g["target"]), int(g["control"]), float(g["theta"]))
+                          for g in rnd10]}
+    for name, gl in circuits.items():
+        s = shots[name]
+        j = m.Job(s["n"]).set_num_shots(s["shots"]).set_num_workers(s["workers"]).set_rng_seed(s["seed"])
+        for t, tg, c, th in gl:
+            j.add_gate(t, tg, c, th if t in (7, 8, 9) else None)
+        assert j.to_json() == s["json"], name
+    # size probe and truncation contract
+    L = m.lib()
+    need = L.moonlab_job_to_json(j.handle, None, 0)
+    small = ctypes.create_string_buffer(16)
+    assert L.moonlab_job_to_json(j.handle, small, 16) == need
+    assert small.value.decode() == s["json"][:15]
+
+
+def test_registry_default_and_stub_dispatch():
+    names = m.list_backends()
+    assert names[0] == "simulator"
+    L = m.lib()
+    sim = L.moonlab_find_backend(b"simulator")
+    assert sim and sim.contents.name == b"simulator"
+    hits = []
+
+    @m.BACKEND_EXECUTE_FN
+    def stub(job, out, ctx):
+        hits.append(1)
+        n = L.moonlab_job_num_shots(job)
+        libc = ctypes.CDLL(None)
+        libc.malloc.restype = ctypes.c_void_p
+        libc.calloc.restype = ctypes.c_void_p
+        out.contents.num_qubits = L.moonlab_job_num_qubits(job)
+        out.contents.total_shots = n
+        out.contents.num_workers_used = 1
+        out.contents.outcomes = ctypes.cast(libc.malloc(8 * n), ctypes.POINTER(ctypes.c_uint64))
+        out.contents.worker_seconds = ctypes.cast(libc.calloc(1, 8), ctypes.POINTER(ctypes.c_double))
+        for i in range(n):
+            out.contents.outcomes[i] = 0xA5
+        return 0
+
+    be = m._Backend(b"test-stub", stub, None, b"stub")
+    assert L.moonlab_register_backend(ctypes.byref(be)) == 0
+    assert L.moonlab_find_backend(b"test-stub")
+    n0 = L.moonlab_num_backends()
+    assert L.moonlab_register_backend(ctypes.byref(be)) == 0  # same name replaces
+    assert L.moonlab_num_backends() == n0
+    j = m.Job(2).add_gate(4, 0).set_num_shots(64).set_backend("test-stub")
+    assert j.backend == "test-stub"
+    r = j.execute()
+    assert len(hits) == 1 and r.total_shots == 64 and set(r.outcomes) == {0xA5}
+    assert L.moonlab_unregister_backend(b"test-stub") == 0
+    assert not L.moonlab_find_backend(b"test-stub")
+    assert L.moonlab_unregister_backend(b"test-stub") == m.MOONLAB_SCHED_BACKEND_NOT_FOUND
+    j.set_backend("does-not-exist")
+    rc, _ = j.run_raw()
+    assert rc == m.MOONLAB_SCHED_BACKEND_NOT_FOUND
+    j.set_backend(None)
+    assert j.backend is None
+    assert L.moonlab_register_backend(None) == m.MOONLAB_SCHED_BAD_ARG
+
+
+def test_registry_capacity_is_sixteen():
+    L = m.lib()
+
+    @m.BACKEND_EXECUTE_FN
+    def nop(job, out, ctx):
+        return m.MOONLAB_SCHED_INTERNAL
+
+    base = L.moonlab_num_backends()
+    keep = []
+    added = []
+    try:
+        for i in range(32):
+            name = f"cap-{i}".encode()
+            be = m._Backend(name, nop, None, None)
+            keep.append((be, name))
+            rc = L.moonlab_register_backend(ctypes.byref(be))
+            if rc != 0:
+                assert rc == m.MOONLAB_SCHED_OOM
+                break
+            added.append(name)
+        assert base + len(added) == 16
+    finally:
+        for name in added:
+            L.moonlab_unregister_backend(name)
+
+
+def test_completion_hook_and_request_context():
+    L = m.lib()
+    seen = []
+
+    @m.COMPLETION_HOOK_FN
+    def hook(job, res, backend, ctx):
+        seen.append((backend.decode(), res.contents.total_shots,
+                     L.moonlab_scheduler_current_tenant_id()))
+
+    @m.BACKEND_EXECUTE_FN
+    def ok(job, out, ctx):
+        libc = ctypes.CDLL(None)
+        libc.calloc.restype = ctypes.c_void_p
+        n = L.moonlab_job_num_shots(job)
+        out.contents.total_shots = n
+        out.contents.num_workers_used = 1
+        out.contents.outcomes = ctypes.cast(libc.calloc(n, 8), ctypes.POINTER(ctypes.c_uint64))
+        out.contents.worker_seconds = ctypes.cast(libc.calloc(1, 8), ctypes.POINTER(ctypes.c_double))
+        return 0
+
+    @m.BACKEND_EXECUTE_FN
+    def bad(job, out, ctx):
+        return m.MOONLAB_SCHED_INTERNAL
+
+    b1 = m._Backend(b"hook-ok", ok, None, None)
+    b2 = m._Backend(b"hook-bad", bad, None, None)
+    L.moonlab_register_backend(ctypes.byref(b1))
+    L.moonlab_register_backend(ctypes.byref(b2))
+    try:
+        assert L.moonlab_scheduler_set_completion_hook(hook, None) == 0
+        L.moonlab_scheduler_set_request_context(b"acme-corp", b"req-42")
+        m.Job(2).set_num_shots(8).set_backend("hook-ok").execute()
+        assert seen == [("hook-ok", 8, b"acme-corp")]
+        rc, _ = m.Job(2).set_num_shots(8).set_backend("hook-bad").run_raw()
+        assert rc == m.MOONLAB_SCHED_INTERNAL and len(seen) == 1  # not fired on failure
+        # request context is per thread
+        other = []
+        t = threading.Thread(target=lambda: other.append(L.moonlab_scheduler_current_tenant_id()))
+        t.start()
+        t.join()
+        assert other == [None]
+        L.moonlab_scheduler_set_request_context(None, None)
+        L.moonlab_scheduler_set_completion_hook(m.COMPLETION_HOOK_FN(), None)
+        m.Job(2).set_num_shots(8).set_backend("hook-ok").execute()
+        assert len(seen) == 1
+    finally:
+        L.moonlab_scheduler_set_completion_hook(m.COMPLETION_HOOK_FN(), None)
+        L.moonlab_unregister_backend(b"hook-ok")
+        L.moonlab_unregister_backend(b"hook-bad")
+
+
+def test_invalid_qubit_surfaces_as_internal_before_any_device_work():
+    """A bad index is an INTERNAL error at run time (scheduler.c:159-166);
+    the engine rejects it before touching the GPU."""
+    j = m.Job(2).add_gate(m.GateType.H, 5).set_num_shots(4)
+    rc, _ = j.run_raw()
+    assert rc == m.MOONLAB_SCHED_INTERNAL
+    j = m.Job(2).add_gate(m.GateType.CNOT, 1, 1).set_num_shots(4).set_backend(None)
+    rc, _ = j.run_raw()
+    assert rc == m.MOONLAB_SCHED_INTERNAL
+
+
+def test_circuit_api_validation_and_text_format():
+    L = m.lib()
+    assert not L.moonlab_qgtl_circuit_create(0)
+    assert not L.moonlab_qgtl_circuit_create(33)
+    c = m.Circuit(2)
+    assert c.add_gate_raw(m.GateType.H, 2) == m.MOONLAB_QGTL_BAD_ARG
+    assert c.add_gate_raw(m.GateType.CNOT, 0, 0) == m.MOONLAB_QGTL_BAD_ARG
+    assert c.add_gate_raw(m.GateType.RX, 0) == m.MOONLAB_QGTL_BAD_ARG
+    L.moonlab_qgtl_circuit_free(None)
+    shots = json.load(open(os.path.join(GOLDEN, "shots_ref.json")))
+    rnd10 = np.load(os.path.join(GOLDEN, "rnd10_gates.npy"))
+    c = m.Circuit(10)
+    for g in rnd10:
+        t = int(g["type"])
+        c.add_gate(t, int(g["target"]), int(g["control"]), float(g["theta"]) if t in (7, 8, 9) else None)
+    text = c.serialize()
+    assert text == shots["qgtl_rnd10"]["text"]
+    c2 = m.Circuit.deserialize(text)
+    assert c2.serialize() == text and c2.num_gates == c.num_gates
+    # parser edge cases (moonlab_qgtl_backend.c:414-506)
+    ok = "# c\n\n   NUM_QUBITS 3\n# gate\nH 0\n  CNOT 1 0\nRZ 2 0.25\n"
+    assert m.Circuit.deserialize(ok).num_gates == 3
+    for bad in ["", "H 0\n", "NUM_QUBITS 0\n", "NUM_QUBITS 33\n", "NUM_QUBITS 2\nFOO 0\n",
+                "NUM_QUBITS 2\nH 2\n", "NUM_QUBITS 2\nRX 0\n", "NUM_QUBITS 2\nCNOT 0 0\n",
+                "NUM_QUBITS 2\n" + "H" * 300 + "\n"]:
+        with pytest.raises(m.QGTLError):
+            m.Circuit.deserialize(bad)
+    st = ctypes.c_int(0)
+    payload = b"NUM_QUBITS 2\nH 0\x00\n"
+    assert not L.moonlab_qgtl_circuit_deserialize(payload, len(payload), ctypes.byref(st))
+    assert st.value == m.MOONLAB_QGTL_BAD_ARG
+    # size query and undersized buffer
+    need = ctypes.c_size_t(0)
+    assert L.moonlab_qgtl_circuit_serialize(c._h, None, 0, ctypes.byref(need)) == 0
+    assert need.value == len(text)
+    buf = ctypes.create_string_buffer(10)
+    w = ctypes.c_size_t(0)
+    assert L.moonlab_qgtl_circuit_serialize(c._h, buf, 10, ctypes.byref(w)) == m.MOONLAB_QGTL_OOM
+
+
+def test_circuit_save_load_roundtrip(tmp_path):
+    c = m.Circuit(3).add_gate(m.GateType.H, 0).add_gate(m.GateType.RY, 1, theta=0.1234567890123)
+    p = str(tmp_path / "c.txt").encode()
+    assert m.lib().moonlab_qgtl_circuit_save(c._h, p) == 0
+    st = ctypes.c_int(-1)
+    h = m.lib().moonlab_qgtl_circuit_load(p, ctypes.byref(st))
+    assert h and st.value == 0
+    assert m.Circuit(_handle=h).serialize() == c.serialize()
+
+
+def test_no_device_fails_loudly():
+    """Without a visible GPU the engine refuses to run: there is no CPU path."""
+    if m.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(m.EngineError, match="no HIP device"):
+        m.StateVector(4)
+    rc, _ = m.Job(2).add_gate(4, 0).set_num_shots(4).run_raw()
+    assert rc == m.MOONLAB_SCHED_INTERNAL

@@ -1,0 +1,342 @@
+"""HIP engine parity against the CPU oracle and the reference's goldens.
+
+Tolerances: complex128 amplitudes and FP64 reductions within 1e-12 of the
+oracle (absolute, per amplitude / per value); complex64 states within
+5e-6 per amplitude and fidelity >= 1 - 1e-6 (FP32 storage, O(gates * 2^-24)
+error).  Shot outcomes are compared exactly (same uniform streams, same
+outcome rule) at n <= 20.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_corpus, random_gate_list, random_state
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL64 = 5e-6
+
+
+def _bytes_equal(a, b):
+    return a.tobytes() == b.tobytes()
+
+
+def test_every_gate_code_vs_oracle(eng, oracle):
+    rng = np.random.default_rng(1)
+    n = 9
+    for t in list(range(1, 25)):
+        for trial in range(3):
+            a0 = random_state(n, rng)
+            qs = rng.permutation(n)[:3]
+            g = [(t, int(qs[0]), int(qs[1]), int(qs[2]), float(rng.uniform(-7, 7)),
+                  float(rng.uniform(-7, 7)), float(rng.uniform(-7, 7)))]
+            want = oracle.apply(n, a0.copy(), g)
+            with eng.StateVector(n) as s:
+                s.load(a0).apply(g)
+                got = s.amplitudes()
+            assert np.max(np.abs(got - want)) < TOL, (t, trial)
+            with eng.StateVector(n, eng.C64) as s:
+                s.load(a0).apply(g)
+                got = s.amplitudes()
+            assert np.max(np.abs(got - want)) < TOL64, (t, trial, "c64")
+
+
+def test_every_target_position_and_tuning(eng, oracle):
+    """All target/control positions (incl. 0, 1, n-1) and every launch shape
+    give bit-identical results."""
+    rng = np.random.default_rng(2)
+    n = 12
+    a0 = random_state(n, rng)
+    gl = []
+    for q in range(n):
+        gl += [(4, q), (7, q, -1, -1, 0.3 + q), (9, q, -1, -1, 1.1 * q), (6, q)]
+    for c in range(n):
+        for t in (0, 1, n - 1, (c + 5) % n):
+            if t != c:
+                gl += [(10, t, c), (12, t, c), (13, t, c), (11, t, c)]
+    want = oracle.apply(n, a0.copy(), gl)
+    ref_bytes = None
+    for ept in (1, 2, 4, 8):
+        for block in (64, 256, 1024):
+            eng.set_gate_tuning(ept, block)
+            with eng.StateVector(n) as s:
+                got = s.load(a0).apply(gl).amplitudes()
+            assert np.max(np.abs(got - want)) < TOL, (ept, block)
+            if ref_bytes is None:
+                ref_bytes = got
+            assert _bytes_equal(got, ref_bytes), (ept, block)
+    eng.set_gate_tuning(4, 256)
+
+
+def test_matrix_entry_points_vs_oracle(eng, oracle):
+    import scipy.linalg
+    rng = np.random.default_rng(3)
+    n = 8
+
+    def unitary(k):
+        h = rng.normal(size=(k, k)) + 1j * rng.normal(size=(k, k))
+        return scipy.linalg.expm(1j * (h + h.conj().T))
+
+    for _ in range(6):
+        a0 = random_state(n, rng)
+        q0, q1, q2 = [int(x) for x in rng.permutation(n)[:3]]
+        m2, m4 = unitary(2), unitary(4)
+        want = oracle.apply_matrix_1q(n, a0.copy(), q0, m2)
+        with eng.StateVector(n) as s:
+            assert np.max(np.abs(s.load(a0).apply_1q(q0, m2).amplitudes() - want)) < TOL
+        # moonlab_hip_apply_2q basis |q1 q0>, q0 low == oracle's (qubit1 = q0)
+        want = oracle.apply_matrix_2q(n, a0.copy(), q0, q1, m4)
+        with eng.StateVector(n) as s:
+            assert np.max(np.abs(s.load(a0).apply_2q(q0, q1, m4).amplitudes() - want)) < TOL
+        # multi-control X / Z against Toffoli / CZ compositions
+        want = oracle.apply(n, a0.copy(), [(23, q2, q0, q1)])
+        with eng.StateVector(n) as s:
+            got = s.load(a0).apply_mcx((1 << q0) | (1 << q1), q2).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL
+        want = oracle.apply(n, a0.copy(), [(12, q1, q0)])
+        with eng.StateVector(n) as s:
+            got = s.load(a0).apply_mcz((1 << q0) | (1 << q1)).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL
+        # controlled 2x2 == controlled-RY through the gate table
+        th = float(rng.uniform(-3, 3))
+        ry = np.array([[np.cos(th / 2), -np.sin(th / 2)], [np.sin(th / 2), np.cos(th / 2)]])
+        want = oracle.apply(n, a0.copy(), [(20, q1, q0, -1, th)])
+        with eng.StateVector(n) as s:
+            got = s.load(a0).apply_controlled_1q(1 << q0, q1, ry).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL
+
+
+def test_reference_corpus(eng):
+    worst = 0.0
+    for key, n, g, amps, z in load_corpus():
+        with eng.StateVector(n) as s:
+            got = s.apply(g).amplitudes()
+            zz, nrm = s.expect_z_all()
+        worst = max(worst, np.max(np.abs(got - amps)))
+        assert np.max(np.abs(zz - z)) < TOL, key
+        assert abs(nrm - 1.0) < TOL
+    assert worst < TOL
+
+
+def test_random_full_table_circuits(eng, oracle):
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 3, 5, 11, 12, 13, 16):
+        gl = random_gate_list(n, 120, rng)
+        want = oracle.run(n, gl)
+        with eng.StateVector(n) as s:
+            got = s.apply(gl).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL, n
+
+
+def test_layers20_against_reference_and_oracle(eng, oracle):
+    """The 20-qubit, depth-20 random-layer circuit (780 gates)."""
+    f = np.load(os.path.join(GOLDEN, "layers20_ref.npz"))
+    n = 20
+    g = f["gates"]
+    want = oracle.run(n, g)
+    with eng.StateVector(n) as s:
+        got = s.apply(g).amplitudes()
+        z, nrm = s.expect_z_all()
+        p = s.probabilities()
+    assert np.max(np.abs(got - want)) < TOL
+    assert np.max(np.abs(got[f["sample_idx"]] - f["sample_amps"])) < TOL
+    assert np.max(np.abs(z - f["z"])) < TOL
+    assert np.max(np.abs(z - oracle.expect_z_all(n, want))) < TOL
+    assert np.max(np.abs(p - oracle.probabilities(n, want))) < 1e-15
+    assert abs(nrm - 1.0) < TOL
+
+
+def test_reductions_vs_oracle(eng, oracle):
+    rng = np.random.default_rng(6)
+    for n in (1, 2, 3, 7, 11, 12, 13, 15):
+        a = random_state(n, rng)
+        a *= 1.0 + 0.1 * rng.uniform()   # unnormalised on purpose
+        with eng.StateVector(n) as s:
+            s.load(a)
+            z, nrm = s.expect_z_all()
+            assert np.max(np.abs(z - oracle.expect_z_all(n, a))) < TOL, n
+            assert abs(nrm - oracle.norm_sq(n, a)) < TOL
+            for q in range(n):
+                assert abs(s.prob_one(q) - oracle.L.svo_probability_one(n, a.view(np.float64).ctypes.data_as(
+                    __import__("ctypes").POINTER(__import__("ctypes").c_double)), q)) < TOL
+                assert abs(s.expect_x(q) - oracle.expect_x(n, a, q)) < TOL
+                assert abs(s.expect_y(q) - oracle.expect_y(n, a, q)) < TOL
+            for q in range(n - 1):
+                assert abs(s.correlation_zz(q, q + 1) - oracle.correlation_zz(n, a, q, q + 1)) < TOL
+            b = random_state(n, rng)
+            with eng.StateVector(n) as t:
+                t.load(b)
+                ip = s.inner(t)
+            assert abs(ip - np.vdot(a, b)) < TOL
+            s.normalize()
+            assert abs(s.norm() - 1.0) < TOL
+
+
+def test_reductions_are_bit_reproducible(eng):
+    rng = np.random.default_rng(8)
+    n = 18
+    a = random_state(n, rng)
+    with eng.StateVector(n) as s:
+        s.load(a)
+        z1, n1 = s.expect_z_all()
+        z2, n2 = s.expect_z_all()
+    assert _bytes_equal(z1, z2) and n1 == n2
+
+
+def test_sampler_matches_collapse_rule(eng, oracle):
+    rng = np.random.default_rng(9)
+    for n in (1, 3, 8, 11, 12, 13, 16):
+        a = random_state(n, rng)
+        a[: (1 << n) // 8] = 0.0        # a zero stretch
+        a[-3:] = 0.0                     # trailing zeros: the fallback must skip them
+        a /= np.linalg.norm(a)
+        u = np.concatenate([rng.uniform(size=400), [0.0, 1e-300, 1.0 - 2 ** -53, 1.0, 2.0]])
+        want = np.array([oracle.measure_all_outcome(n, a, x) for x in u], dtype=np.uint64)
+        with eng.StateVector(n) as s:
+            got = s.load(a).sample(u)
+        assert np.array_equal(got, want), n
+
+
+def test_job_shots_bit_exact_vs_reference(eng):
+    shots = json.load(open(os.path.join(GOLDEN, "shots_ref.json")))
+    rnd10 = np.load(os.path.join(GOLDEN, "rnd10_gates.npy"))
+    circuits = {"bell": [(4, 0, -1, 0.0), (10, 1, 0, 0.0)],
+                "ghz": [(4, 0, -1, 0.0), (10, 1, 0, 0.0), (10, 2, 1, 0.0)],
+                "rnd10": [(int(g["type"]), int(g["target"]), int(g["control"]), float(g["theta"])) for g in rnd10]}
+    circuits["rnd10_w1"] = circuits["rnd10"]
+    eng.register_mi355x_backend()
+    for backend in (None, "mi355x"):
+        for name, gl in circuits.items():
+            s = shots[name]
+            j = eng.Job(s["n"]).set_num_shots(s["shots"]).set_num_workers(s["workers"]).set_rng_seed(s["seed"])
+            for t, tg, c, th in gl:
+                j.add_gate(t, tg, c, th if t in (7, 8, 9) else None)
+            j.set_backend(backend)
+            r = j.execute()
+            assert r.outcomes == s["outcomes"], (backend, name)
+            assert r.num_workers_used == s["used"]
+            assert len(r.worker_seconds) == s["used"]
+    q = shots["qgtl_rnd10"]
+    c = eng.Circuit(10)
+    for t, tg, ctl, th in circuits["rnd10"]:
+        c.add_gate(t, tg, ctl, th if t in (7, 8, 9) else None)
+    r = c.execute(num_shots=q["shots"], rng_seed=q["seed"], return_probabilities=True)
+    assert r.outcomes.tolist() == q["outcomes"]
+    assert np.max(np.abs(r.probabilities - np.array(q["probabilities"]))) < 1e-15
+
+
+def test_reference_unit_cases(eng):
+    """tests/unit/test_qgtl_backend.c and test_scheduler.c known answers."""
+    c = eng.Circuit(2).add_gate(4, 0).add_gate(10, 1, 0)
+    p = c.execute(return_probabilities=True).probabilities
+    assert abs(p[0] - 0.5) < 1e-9 and abs(p[3] - 0.5) < 1e-9 and p[1] < 1e-9 and p[2] < 1e-9
+    r = c.execute(num_shots=1024, rng_seed=0xdeadbeef)
+    assert set(r.outcomes.tolist()) <= {0, 3}
+    n00 = int(np.sum(r.outcomes == 0))
+    assert 512 - 80 < n00 < 512 + 80
+    g = eng.Circuit(2)
+    for t, tg, ctl in [(4, 0, -1), (4, 1, -1), (12, 1, 0), (4, 0, -1), (4, 1, -1), (3, 0, -1), (3, 1, -1),
+                       (12, 1, 0), (4, 0, -1), (4, 1, -1)]:
+        g.add_gate(t, tg, ctl)
+    assert g.execute(return_probabilities=True).probabilities[3] > 0.99
+    ry = eng.Circuit(1).add_gate(8, 0, theta=np.pi / 2).execute(return_probabilities=True).probabilities
+    assert abs(ry[0] - 0.5) < 1e-9
+    j = eng.Job(3).add_gate(4, 0).add_gate(10, 1, 0).add_gate(10, 2, 1)
+    r = j.set_num_shots(3000).set_num_workers(3).set_rng_seed(0xCAFE).execute()
+    assert set(r.outcomes) <= {0, 7} and len(r.outcomes) == 3000
+
+
+def test_execute_ex_outputs(eng, oracle):
+    rng = np.random.default_rng(10)
+    n = 14
+    from moonlab_amd import workloads
+    g = workloads.random_layer_circuit(n, 8, 77)
+    j = eng.Job(n).set_num_shots(500).set_num_workers(3).set_rng_seed(0xABCDEF)
+    for row in g:
+        t = int(row["type"])
+        j.add_gate(t, int(row["target"]), int(row["control"]), float(row["theta"]) if t in (7, 8, 9) else None)
+    want = oracle.run(n, g)
+    r = j.execute_ex(amplitudes=True, probabilities=True, expect_z=True)
+    assert np.max(np.abs(r.amplitudes - want)) < TOL
+    assert np.max(np.abs(r.expect_z - oracle.expect_z_all(n, want))) < TOL
+    assert np.max(np.abs(r.probabilities - np.abs(want) ** 2)) < 1e-15
+    assert r.outcomes.tolist() == oracle.job_shots(n, want, 500, 3, 0xABCDEF).tolist()
+    r64 = j.execute_ex(precision=eng.C64, amplitudes=True, expect_z=True)
+    fid = abs(np.vdot(want, r64.amplitudes)) ** 2
+    assert fid > 1 - 1e-6
+    assert np.max(np.abs(r64.expect_z - oracle.expect_z_all(n, want))) < 1e-5
+    rf = j.execute_ex(amplitudes=True, fusion_max_qubits=4)
+    assert np.max(np.abs(rf.amplitudes - want)) < TOL
+
+
+def test_fused_application_vs_unfused(eng, oracle):
+    rng = np.random.default_rng(12)
+    from moonlab_amd import workloads
+    for n, k in ((6, 3), (10, 4), (14, 5), (16, 5), (13, 2), (9, 1)):
+        g = workloads.su2_cz_circuit(n, 6, 1000 + n)
+        want = oracle.run(n, g)
+        with eng.StateVector(n) as s:
+            got = s.apply(g, fuse=k).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL, (n, k)
+        with eng.StateVector(n, eng.C64) as s:
+            got = s.apply(g, fuse=k).amplitudes()
+        assert abs(np.vdot(want, got)) ** 2 > 1 - 1e-6, (n, k)
+    # a general dense block at scattered qubits
+    import scipy.linalg
+    n = 11
+    for qs in ([0, 1, 2, 3, 4], [1, 4, 6, 9, 10], [0, 5, 10], [2, 3, 7, 8]):
+        k = len(qs)
+        h = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        u = scipy.linalg.expm(1j * (h + h.conj().T))
+        a0 = random_state(n, rng)
+        # reference: brute force in numpy on the reshaped state
+        t = a0.reshape([2] * n)  # axis i = qubit n-1-i
+        axes = [n - 1 - q for q in qs[::-1]]
+        moved = np.moveaxis(t, axes, list(range(k))).reshape(1 << k, -1)
+        want = np.moveaxis((u @ moved).reshape([2] * n), list(range(k)), axes).reshape(-1)
+        with eng.StateVector(n) as s:
+            got = s.load(a0).apply_dense(qs, u).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL, qs
+
+
+def test_hea_tfim_energy_against_reference(eng):
+    rows = json.load(open(os.path.join(GOLDEN, "hea_tfim_ref.json")))
+    for r in rows:
+        n, L, h = r["n"], r["layers"], r["h"]
+        ang = np.array(r["angles"]).reshape(L, n, 2)
+        gl = []
+        for l in range(L):
+            for q in range(n):
+                gl.append((8, q, -1, -1, float(ang[l, q, 0])))
+                gl.append((9, q, -1, -1, float(ang[l, q, 1])))
+            for q in range(n - 1):
+                gl.append((12, q + 1, q))
+        with eng.StateVector(n) as s:
+            s.apply(gl)
+            e = -sum(s.correlation_zz(q, q + 1) for q in range(n - 1)) - h * sum(s.expect_x(q) for q in range(n))
+        assert abs(e - r["energy"]) < TOL
+
+
+def test_large_state_properties(eng):
+    """At 26 qubits (1 GiB c128): a random-layer circuit followed by its
+    inverse returns |0..0>; the norm stays 1 throughout."""
+    from moonlab_amd import workloads
+    n = 26
+    g = workloads.random_layer_circuit(n, 4, 5, h_layer=True)
+    inv = []
+    for row in g[::-1]:
+        t = int(row["type"])
+        th = -float(row["theta"]) if t in (7, 8, 9) else 0.0
+        inv.append((t, int(row["target"]), int(row["control"]), -1, th))
+    with eng.StateVector(n) as s:
+        s.apply(g)
+        _, nrm = s.expect_z_all()
+        assert abs(nrm - 1.0) < 1e-12
+        s.apply(inv)
+        head = s.read(0, 4)
+        z, nrm = s.expect_z_all()
+    assert abs(head[0] - 1.0) < 1e-12 and np.max(np.abs(head[1:])) < 1e-12
+    assert np.max(np.abs(z - 1.0)) < 1e-12
