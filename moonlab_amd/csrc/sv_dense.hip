@@ -41,8 +41,8 @@ __global__ __launch_bounds__(256) void k_dense(cplx_t<R> *__restrict__ a, uint64
 #pragma unroll
         for (int c = 0; c < D; c++) {
             const cplx_t<R> mm = m[r * D + c];
-            sr += mm.x * v[c].x - mm.y * v[c].y;
-            si += mm.x * v[c].y + mm.y * v[c].x;
+            sr = fmar(mm.x, v[c].x, fmar(-mm.y, v[c].y, sr));
+            si = fmar(mm.x, v[c].y, fmar(mm.y, v[c].x, si));
         }
         a[off[r]] = cplx_t<R>{sr, si};
     }

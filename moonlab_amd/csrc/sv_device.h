@@ -15,6 +15,23 @@ template <> struct Cplx<double> { using type = double2; };
 template <> struct Cplx<float> { using type = float2; };
 template <typename R> using cplx_t = typename Cplx<R>::type;
 
+// Fused multiply-add with the operation order fixed in the source, so the
+// compiler's contraction choices cannot differ between kernel instantiations
+// (launch shapes, tile sizes): every kernel that applies the same operator
+// produces bit-identical amplitudes.
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Complex multiply (x.re, x.im) * (pr, pi) with a fixed FMA pattern.
+template <typename T, typename R>
+__device__ __forceinline__ T cmul(const T &x, R pr, R pi)
+{
+    T y;
+    y.x = fmar(x.x, pr, -(x.y * pi));
+    y.y = fmar(x.x, pi, x.y * pr);
+    return y;
+}
+
 // Insert a zero bit at position q of p (pair / subspace enumeration).
 __device__ __forceinline__ uint64_t ins0(uint64_t p, int q)
 {
@@ -82,8 +99,9 @@ template <typename R> struct OpRX {              // [[c,-is],[-is,c]] (gates.c:3
     __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
     {
         const cplx_t<R> a0 = a;
-        a.x = c * a0.x + s * b.y;  a.y = c * a0.y - s * b.x;
-        b.x = c * b.x + s * a0.y;  b.y = c * b.y - s * a0.x;
+        const cplx_t<R> b0 = b;
+        a.x = fmar(c, a0.x, s * b0.y);  a.y = fmar(c, a0.y, -(s * b0.x));
+        b.x = fmar(c, b0.x, s * a0.y);  b.y = fmar(c, b0.y, -(s * a0.x));
     }
 };
 
@@ -92,8 +110,9 @@ template <typename R> struct OpRY {              // [[c,-s],[s,c]] (gates.c:429-
     __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
     {
         const cplx_t<R> a0 = a;
-        a.x = c * a0.x - s * b.x;  a.y = c * a0.y - s * b.y;
-        b.x = s * a0.x + c * b.x;  b.y = s * a0.y + c * b.y;
+        const cplx_t<R> b0 = b;
+        a.x = fmar(c, a0.x, -(s * b0.x));  a.y = fmar(c, a0.y, -(s * b0.y));
+        b.x = fmar(s, a0.x, c * b0.x);     b.y = fmar(s, a0.y, c * b0.y);
     }
 };
 
@@ -101,9 +120,8 @@ template <typename R> struct OpDiag2 {           // diag(d0, d1) on a pair (cont
     R d0r, d0i, d1r, d1i;
     __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
     {
-        const cplx_t<R> a0 = a, b0 = b;
-        a.x = a0.x * d0r - a0.y * d0i; a.y = a0.x * d0i + a0.y * d0r;
-        b.x = b0.x * d1r - b0.y * d1i; b.y = b0.x * d1i + b0.y * d1r;
+        a = cmul(a, d0r, d0i);
+        b = cmul(b, d1r, d1i);
     }
 };
 
@@ -112,10 +130,10 @@ template <typename R> struct OpMat2 {            // general 2x2, row-major (re, 
     __device__ __forceinline__ void operator()(cplx_t<R> &a, cplx_t<R> &b) const
     {
         const cplx_t<R> a0 = a, b0 = b;
-        a.x = (m[0] * a0.x - m[1] * a0.y) + (m[2] * b0.x - m[3] * b0.y);
-        a.y = (m[0] * a0.y + m[1] * a0.x) + (m[2] * b0.y + m[3] * b0.x);
-        b.x = (m[4] * a0.x - m[5] * a0.y) + (m[6] * b0.x - m[7] * b0.y);
-        b.y = (m[4] * a0.y + m[5] * a0.x) + (m[6] * b0.y + m[7] * b0.x);
+        a.x = fmar(m[0], a0.x, -(m[1] * a0.y)) + fmar(m[2], b0.x, -(m[3] * b0.y));
+        a.y = fmar(m[0], a0.y, m[1] * a0.x) + fmar(m[2], b0.y, m[3] * b0.x);
+        b.x = fmar(m[4], a0.x, -(m[5] * a0.y)) + fmar(m[6], b0.x, -(m[7] * b0.y));
+        b.y = fmar(m[4], a0.y, m[5] * a0.x) + fmar(m[6], b0.y, m[7] * b0.x);
     }
 };
 
@@ -145,8 +163,7 @@ template <typename R> struct ElPhase {           // T, P, CPHASE (gates.c:276-30
     R pr, pi;
     __device__ __forceinline__ void operator()(cplx_t<R> &a) const
     {
-        const R r = a.x, i = a.y;
-        a.x = r * pr - i * pi; a.y = r * pi + i * pr;
+        a = cmul(a, pr, pi);
     }
 };
 

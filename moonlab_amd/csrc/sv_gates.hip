@@ -120,11 +120,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_diag(cplx_t<R> *__restrict__ a, u
         const uint64_t i = base + (uint64_t)k * bs;
         if (!(full || i < dim)) continue;
         const bool one = (i >> q) & 1ull;
-        const R pr = one ? d1r : d0r, pi = one ? d1i : d0i;
-        const R r = x[k].x, m = x[k].y;
-        x[k].x = r * pr - m * pi;
-        x[k].y = r * pi + m * pr;
-        a[i] = x[k];
+        a[i] = cmul(x[k], one ? d1r : d0r, one ? d1i : d0i);
     }
 }
 
@@ -243,8 +239,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_mat4(cplx_t<R> *__restrict__ a, u
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const R mr = M.m[r * 8 + c * 2], mi = M.m[r * 8 + c * 2 + 1];
-                sr += mr * v[c].x - mi * v[c].y;
-                si += mr * v[c].y + mi * v[c].x;
+                sr = fmar(mr, v[c].x, fmar(-mi, v[c].y, sr));
+                si = fmar(mr, v[c].y, fmar(mi, v[c].x, si));
             }
             a[idx[r]] = cplx_t<R>{sr, si};
         }

@@ -42,7 +42,7 @@ template <typename R>
 __device__ __forceinline__ double prob(const cplx_t<R> &x)
 {
     const double r = (double)x.x, i = (double)x.y;
-    return r * r + i * i;
+    return fmar(r, r, i * i);
 }
 
 // n >= 12: tiles of 4096 amplitudes.
@@ -295,8 +295,8 @@ __global__ __launch_bounds__(256) void k_xy(const cplx_t<R> *__restrict__ a, uin
     for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
         const uint64_t i = ins0(p, q), j = i | (1ull << q);
         const double ar = a[i].x, ai = a[i].y, br = a[j].x, bi = a[j].y;
-        v[0] += ar * br + ai * bi;
-        v[1] += ar * bi - ai * br;
+        v[0] += fmar(ar, br, ai * bi);
+        v[1] += fmar(ar, bi, -(ai * br));
     }
     block_store<2>(v, partial);
 }
@@ -323,8 +323,8 @@ __global__ __launch_bounds__(256) void k_inner(const cplx_t<R> *__restrict__ a,
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
         const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
-        v[0] += ar * br + ai * bi;
-        v[1] += ar * bi - ai * br;
+        v[0] += fmar(ar, br, ai * bi);
+        v[1] += fmar(ar, bi, -(ai * br));
     }
     block_store<2>(v, partial);
 }
