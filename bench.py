@@ -82,6 +82,9 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for profiler counters: one device copy of the state "
+                         "(known bytes, the calibration kernel) and the first layer of the sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs on a single GPU "
@@ -100,6 +103,18 @@ def main() -> int:
     per_launch = {"k_pair": 2 * D * 16, "k_diag": 2 * D * 16, "k_ctrl_pair": 2 * (D // 2) * 16}
     print(f"[bench] n={n} gates/step={G} algorithmic={bytes_step / 1e12:.3f} TB/step", file=sys.stderr,
           flush=True)
+
+    if args.probe:
+        a = m.StateVector(n, m.C128)
+        b = m.StateVector(n, m.C128)
+        a.apply(gates[:n])
+        b.copy_from(a)          # k_convert<double, double>: reads and writes 2^n x 16 B
+        b.synchronize()
+        b.apply(gates[n:2 * n + (n - 1)])   # first layer: n rotations + n - 1 CNOTs
+        b.synchronize()
+        print(json.dumps({"probe": True, "qubits": n, "copy_bytes_each_way": D * 16,
+                          "gates": int(2 * n + n - 1)}), flush=True)
+        return 0
 
     st = m.StateVector(n, m.C128)
     L = m.lib()

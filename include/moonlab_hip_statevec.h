@@ -212,6 +212,11 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_state_copy_from_host(moonlab_hip_st
 MOONLAB_API moonlab_hip_status_t moonlab_hip_state_read(const moonlab_hip_state_t *state,
                                                         uint64_t offset, uint64_t count,
                                                         void *out);
+/* Addition: device-side clone (quantum_state_clone, src/quantum/state.c:
+ * 164-191, without a host round trip): dst takes src's amplitudes, converted
+ * when the precisions differ.  Same qubit count required. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_state_copy(moonlab_hip_state_t *dst,
+                                                        const moonlab_hip_state_t *src);
 /* Addition: device pointer and stream (hipStream_t) of the state. */
 MOONLAB_API void *moonlab_hip_state_device_ptr(moonlab_hip_state_t *state);
 MOONLAB_API void *moonlab_hip_state_stream(moonlab_hip_state_t *state);
@@ -240,6 +245,11 @@ MOONLAB_API const char *moonlab_hip_last_error(void);
  * thread (1, 2, 4, 8) and threads per block (64..1024).  0 keeps a value. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_set_gate_tuning(int pairs_per_thread,
                                                              int block_threads);
+
+/* Addition: states of at least `bytes` use streaming (non-temporal) loads
+ * and stores in the gate kernels; smaller ones keep cached access so
+ * consecutive gates can hit L2 / MALL.  Default 512 MiB. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_set_streaming_threshold(uint64_t bytes);
 
 #ifdef __cplusplus
 }

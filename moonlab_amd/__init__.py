@@ -240,6 +240,8 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_state_copy_from_host": ([vp, dp], ip),
         "moonlab_hip_state_read": ([vp, u64, u64, vp], ip),
         "moonlab_hip_state_device_ptr": ([vp], vp),
+        "moonlab_hip_state_copy": ([vp, vp], ip),
+        "moonlab_hip_set_streaming_threshold": ([u64], ip),
         "moonlab_hip_state_stream": ([vp], vp),
         "moonlab_hip_synchronize": ([vp], ip),
         "moonlab_hip_timer_begin": ([vp, ip], ip),
@@ -592,6 +594,10 @@ class StateVector:
             "apply_controlled_1q")
         return self
 
+    def copy_from(self, other: "StateVector") -> "StateVector":
+        _check_hip(lib().moonlab_hip_state_copy(self._h, other._h), "state_copy")
+        return self
+
     def synchronize(self) -> None:
         _check_hip(lib().moonlab_hip_synchronize(self._h), "synchronize")
 
@@ -684,6 +690,10 @@ class StateVector:
         if n < 0:
             raise EngineError(f"timer_elapsed failed: {last_error()}")
         return out[:n]
+
+
+def set_streaming_threshold(nbytes: int) -> None:
+    _check_hip(lib().moonlab_hip_set_streaming_threshold(nbytes), "set_streaming_threshold")
 
 
 def set_gate_tuning(pairs_per_thread: int = 0, block_threads: int = 0) -> None:

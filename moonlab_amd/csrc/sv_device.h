@@ -32,6 +32,53 @@ __device__ __forceinline__ T cmul(const T &x, R pr, R pi)
     return y;
 }
 
+// Streaming (non-temporal) loads and stores: a gate pass over a state larger
+// than the last-level cache has no reuse, and marking the traffic streaming
+// lets L2 and the MALL skip allocating it.
+typedef double mlx_v2d __attribute__((ext_vector_type(2)));
+typedef float mlx_v2f __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ double2 ld(const double2 *p)
+{
+    if constexpr (NT) {
+        const mlx_v2d v = __builtin_nontemporal_load((const mlx_v2d *)p);
+        return double2{v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ float2 ld(const float2 *p)
+{
+    if constexpr (NT) {
+        const mlx_v2f v = __builtin_nontemporal_load((const mlx_v2f *)p);
+        return float2{v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void st(double2 *p, const double2 &x)
+{
+    if constexpr (NT) {
+        const mlx_v2d v = {x.x, x.y};
+        __builtin_nontemporal_store(v, (mlx_v2d *)p);
+    } else {
+        *p = x;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void st(float2 *p, const float2 &x)
+{
+    if constexpr (NT) {
+        const mlx_v2f v = {x.x, x.y};
+        __builtin_nontemporal_store(v, (mlx_v2f *)p);
+    } else {
+        *p = x;
+    }
+}
+
 // Insert a zero bit at position q of p (pair / subspace enumeration).
 __device__ __forceinline__ uint64_t ins0(uint64_t p, int q)
 {

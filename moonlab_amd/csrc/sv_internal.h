@@ -24,6 +24,7 @@ struct SvDev {
 struct GateTuning {
     int ept;     // 1, 2, 4 or 8
     int block;   // 64, 128, 256, 512 or 1024
+    uint64_t nt_min_bytes;   // states at least this large use streaming loads/stores
 };
 GateTuning &gate_tuning();
 

@@ -161,6 +161,12 @@ moonlab_hip_status_t moonlab_hip_set_gate_tuning(int ept, int block)
     return MOONLAB_HIP_OK;
 }
 
+moonlab_hip_status_t moonlab_hip_set_streaming_threshold(uint64_t bytes)
+{
+    gate_tuning().nt_min_bytes = bytes;
+    return MOONLAB_HIP_OK;
+}
+
 moonlab_hip_status_t moonlab_hip_state_create_ex(uint32_t n, int prec, moonlab_hip_state_t **out)
 {
     if (!out) return fail(MOONLAB_HIP_ERR_BAD_ARG, "out_state is NULL");
@@ -528,6 +534,16 @@ moonlab_hip_status_t moonlab_hip_state_copy_from_host(moonlab_hip_state_t *s, co
         if (e != hipSuccess) return drv(e, "copy_from_host");
     }
     return MOONLAB_HIP_OK;
+}
+
+moonlab_hip_status_t moonlab_hip_state_copy(moonlab_hip_state_t *dst, const moonlab_hip_state_t *src)
+{
+    if (!dst || !src) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");
+    if (dst->n != src->n) return fail(MOONLAB_HIP_ERR_BAD_ARG, "qubit counts differ");
+    if (dst == src) return MOONLAB_HIP_OK;
+    hipError_t e = hipStreamSynchronize(src->stream);
+    if (e == hipSuccess) e = launch_convert(src->amps, src->prec, dst->amps, dst->prec, dst->dim, dst->stream);
+    return drv(e, "state_copy");
 }
 
 moonlab_hip_status_t moonlab_hip_synchronize(moonlab_hip_state_t *s)

@@ -69,6 +69,28 @@ def test_every_target_position_and_tuning(eng, oracle):
                 ref_bytes = got
             assert _bytes_equal(got, ref_bytes), (ept, block)
     eng.set_gate_tuning(4, 256)
+    # streaming (non-temporal) and cached access give the same bits
+    eng.set_streaming_threshold(0)
+    try:
+        with eng.StateVector(n) as s:
+            got = s.load(a0).apply(gl).amplitudes()
+    finally:
+        eng.set_streaming_threshold(512 << 20)
+    assert _bytes_equal(got, ref_bytes)
+
+
+def test_state_copy_and_precision_conversion(eng, oracle):
+    rng = np.random.default_rng(21)
+    n = 13
+    a0 = random_state(n, rng)
+    with eng.StateVector(n) as s, eng.StateVector(n) as t, eng.StateVector(n, eng.C64) as h:
+        s.load(a0)
+        t.copy_from(s)
+        assert _bytes_equal(t.amplitudes(), a0)
+        h.copy_from(s)
+        assert np.max(np.abs(h.amplitudes() - a0)) < 1e-7
+        t.copy_from(h)
+        assert _bytes_equal(t.amplitudes(), a0.astype(np.complex64).astype(np.complex128))
 
 
 def test_matrix_entry_points_vs_oracle(eng, oracle):
