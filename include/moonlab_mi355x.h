@@ -70,6 +70,79 @@ MOONLAB_API void moonlab_mi355x_results_free(moonlab_mi355x_results_t *r);
  * cache. */
 MOONLAB_API int moonlab_mi355x_set_state_cache_bytes(uint64_t bytes);
 
+/* ---- Batched small circuits: one persistent launch -------------------------
+ *
+ * Thousands of independent circuits of uneven size run in ONE kernel launch:
+ * each GPU workgroup takes the next circuit from a device work queue and
+ * simulates it to completion (states up to 12 qubits entirely in LDS; larger
+ * ones cache-blocked through LDS from a per-workgroup HBM slot), then writes
+ * the requested observables and shots.  Results for a circuit do not depend
+ * on what else is in the batch (fixed per-circuit reduction order).
+ * The workload shape is the reference's VQE energy loop
+ * (src/algorithms/vqe.c:1287-1350) over many parameter sets at once. */
+
+typedef struct moonlab_mi355x_batch moonlab_mi355x_batch_t;
+
+#define MOONLAB_MI355X_OBS_Z  1   /* <Z_q> for every qubit (and the norm) */
+#define MOONLAB_MI355X_OBS_ZZ 2   /* <Z_q Z_q+1>, q = 0..n-2 */
+#define MOONLAB_MI355X_OBS_X  4   /* <X_q> for every qubit */
+#define MOONLAB_MI355X_BATCH_MAX_QUBITS 24
+
+typedef struct {
+    int                       num_qubits;   /* 1..24 */
+    int                       precision;    /* MOONLAB_HIP_C128 or MOONLAB_HIP_C64 */
+    const moonlab_hip_gate_t *gates;        /* codes of moonlab_hip_statevec.h */
+    int                       num_gates;
+    int                       observables;  /* MOONLAB_MI355X_OBS_* bits */
+    int                       num_shots;    /* 0 = none */
+    const double             *draws;        /* num_shots uniform draws in [0, 1) */
+    /* outputs, caller-owned; NULL when not requested */
+    double                   *norm;         /* 1 value */
+    double                   *expect_z;     /* n values */
+    double                   *zz_chain;     /* n - 1 values */
+    double                   *expect_x;     /* n values */
+    uint64_t                 *outcomes;     /* num_shots values, the collapse rule of
+                                               measurement_all_qubits (measurement.c:263-307) */
+} moonlab_mi355x_batch_item_t;
+
+typedef struct {
+    double device_seconds;   /* the persistent launch, HIP events on the batch stream */
+    double wall_seconds;     /* the whole call: planning, upload, launch, download */
+    int    workgroups;
+    int    tile_bits;
+    int    total_passes;     /* HBM sweeps planned for the circuits above tile size */
+    int    total_ops;
+} moonlab_mi355x_batch_stats_t;
+
+MOONLAB_API moonlab_mi355x_batch_t *moonlab_mi355x_batch_create(void);
+MOONLAB_API void moonlab_mi355x_batch_free(moonlab_mi355x_batch_t *b);
+/* Workgroups per CU (1 or 2) and tile bits (8..12); 0 keeps a value. */
+MOONLAB_API int moonlab_mi355x_batch_set_config(moonlab_mi355x_batch_t *b, int workgroups_per_cu,
+                                                int tile_bits);
+/* Time every configuration on these items and keep the fastest; the
+ * items' outputs are written by the last timed run. */
+MOONLAB_API int moonlab_mi355x_batch_autotune(moonlab_mi355x_batch_t *b,
+                                              moonlab_mi355x_batch_item_t *items, int count,
+                                              moonlab_mi355x_batch_stats_t *stats);
+MOONLAB_API int moonlab_mi355x_batch_run(moonlab_mi355x_batch_t *b,
+                                         moonlab_mi355x_batch_item_t *items, int count,
+                                         moonlab_mi355x_batch_stats_t *stats);
+
+/* Hardware-efficient ansatz + open-chain transverse-field Ising energy, the
+ * batched form of vqe_compute_energy: per layer RY, RZ on every qubit then
+ * CZ(q, q+1) for q = 0..n-2, from |0..0>; E = -sum <Z_q Z_q+1> - h sum <X_q>. */
+typedef struct {
+    int           num_qubits;
+    int           num_layers;
+    const double *angles;     /* layers * n * 2: [layer][qubit][ry, rz] */
+    double        h;
+} moonlab_mi355x_hea_t;
+
+MOONLAB_API int moonlab_mi355x_hea_tfim_energies(moonlab_mi355x_batch_t *b,
+                                                 const moonlab_mi355x_hea_t *instances, int count,
+                                                 int precision, double *energies,
+                                                 moonlab_mi355x_batch_stats_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

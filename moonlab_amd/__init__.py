@@ -145,6 +145,31 @@ class _ExecResults(ctypes.Structure):
                 ("wall_seconds", ctypes.c_double)]
 
 
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("precision", ctypes.c_int),
+                ("gates", ctypes.c_void_p), ("num_gates", ctypes.c_int),
+                ("observables", ctypes.c_int), ("num_shots", ctypes.c_int),
+                ("draws", ctypes.POINTER(ctypes.c_double)),
+                ("norm", ctypes.POINTER(ctypes.c_double)),
+                ("expect_z", ctypes.POINTER(ctypes.c_double)),
+                ("zz_chain", ctypes.POINTER(ctypes.c_double)),
+                ("expect_x", ctypes.POINTER(ctypes.c_double)),
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64))]
+
+
+class _BatchStats(ctypes.Structure):
+    _fields_ = [("device_seconds", ctypes.c_double), ("wall_seconds", ctypes.c_double),
+                ("workgroups", ctypes.c_int), ("tile_bits", ctypes.c_int),
+                ("total_passes", ctypes.c_int), ("total_ops", ctypes.c_int)]
+
+
+class _Hea(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("num_layers", ctypes.c_int),
+                ("angles", ctypes.POINTER(ctypes.c_double)), ("h", ctypes.c_double)]
+
+
+OBS_Z, OBS_ZZ, OBS_X = 1, 2, 4
+
 _lib: Optional[ctypes.CDLL] = None
 
 
@@ -201,6 +226,12 @@ def _declare(L: ctypes.CDLL) -> None:
                                        ctypes.POINTER(_ExecResults)], ip),
         "moonlab_mi355x_results_free": ([ctypes.POINTER(_ExecResults)], None),
         "moonlab_mi355x_set_state_cache_bytes": ([u64], ip),
+        "moonlab_mi355x_batch_create": ([], vp),
+        "moonlab_mi355x_batch_free": ([vp], None),
+        "moonlab_mi355x_batch_set_config": ([vp, ip, ip], ip),
+        "moonlab_mi355x_batch_run": ([vp, ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_batch_autotune": ([vp, ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_hea_tfim_energies": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
         # state-vector layer
         "moonlab_hip_device_count": ([], ip),
         "moonlab_hip_last_error": ([], ctypes.c_char_p),
@@ -698,3 +729,101 @@ def set_streaming_threshold(nbytes: int) -> None:
 
 def set_gate_tuning(pairs_per_thread: int = 0, block_threads: int = 0) -> None:
     _check_hip(lib().moonlab_hip_set_gate_tuning(pairs_per_thread, block_threads), "set_gate_tuning")
+
+
+# ---------------------------------------------------------------------------
+# Batched small circuits (one persistent launch)
+# ---------------------------------------------------------------------------
+@dataclass
+class BatchResult:
+    norm: float
+    expect_z: Optional[np.ndarray]
+    zz_chain: Optional[np.ndarray]
+    expect_x: Optional[np.ndarray]
+    outcomes: Optional[np.ndarray]
+
+
+class Batch:
+    """moonlab_mi355x_batch_*: many circuits in one persistent launch."""
+
+    def __init__(self) -> None:
+        self._h = lib().moonlab_mi355x_batch_create()
+        if not self._h:
+            raise EngineError("moonlab_mi355x_batch_create failed (no HIP device?)")
+        self.last_stats = None
+
+    def free(self) -> None:
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.moonlab_mi355x_batch_free(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        self.free()
+
+    def set_config(self, workgroups_per_cu: int = 0, tile_bits: int = 0) -> None:
+        _check_sched(lib().moonlab_mi355x_batch_set_config(self._h, workgroups_per_cu, tile_bits), "set_config")
+
+    @staticmethod
+    def _stats(st):
+        return {"device_seconds": st.device_seconds, "wall_seconds": st.wall_seconds,
+                "workgroups": st.workgroups, "tile_bits": st.tile_bits,
+                "total_passes": st.total_passes, "total_ops": st.total_ops}
+
+    def run(self, circuits, autotune: bool = False) -> List[BatchResult]:
+        """circuits: dicts with n, gates, precision (C128), observables (OBS_* bits),
+        draws (uniforms for shots, optional)."""
+        keep = []
+        items = (_BatchItem * len(circuits))()
+        outs = []
+        for i, c in enumerate(circuits):
+            n = int(c["n"])
+            g = gate_array(c.get("gates", []))
+            obs = int(c.get("observables", OBS_Z))
+            draws = np.ascontiguousarray(c.get("draws", np.zeros(0)), dtype=np.float64)
+            norm = np.zeros(1)
+            z = np.zeros(n)
+            zz = np.zeros(max(n - 1, 1))
+            x = np.zeros(n)
+            oc = np.zeros(max(draws.size, 1), dtype=np.uint64)
+            keep += [g, draws, norm, z, zz, x, oc]
+            dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            it = items[i]
+            it.num_qubits = n
+            it.precision = int(c.get("precision", C128))
+            it.gates = g.ctypes.data if len(g) else None
+            it.num_gates = len(g)
+            it.observables = obs
+            it.num_shots = int(draws.size)
+            it.draws = dp(draws) if draws.size else None
+            it.norm = dp(norm)
+            it.expect_z = dp(z) if obs & OBS_Z else None
+            it.zz_chain = dp(zz) if obs & OBS_ZZ else None
+            it.expect_x = dp(x) if obs & OBS_X else None
+            it.outcomes = oc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if draws.size else None
+            outs.append((norm, z if obs & OBS_Z else None, zz[:max(n - 1, 0)] if obs & OBS_ZZ else None,
+                         x if obs & OBS_X else None, oc[:draws.size] if draws.size else None))
+        st = _BatchStats()
+        fn = lib().moonlab_mi355x_batch_autotune if autotune else lib().moonlab_mi355x_batch_run
+        _check_sched(fn(self._h, items, len(circuits), ctypes.byref(st)), "batch_run")
+        self.last_stats = self._stats(st)
+        return [BatchResult(float(o[0][0]), o[1], o[2], o[3], o[4]) for o in outs]
+
+    def hea_tfim(self, instances, precision: int = C128) -> np.ndarray:
+        """instances: (n, layers, angles[layers, n, 2], h) tuples -> energies."""
+        arr = (_Hea * len(instances))()
+        keep = []
+        for i, (n, L, ang, h) in enumerate(instances):
+            a = np.ascontiguousarray(ang, dtype=np.float64).reshape(-1)
+            assert a.size == L * n * 2
+            keep.append(a)
+            arr[i].num_qubits = n
+            arr[i].num_layers = L
+            arr[i].angles = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            arr[i].h = h
+        e = np.zeros(len(instances))
+        st = _BatchStats()
+        _check_sched(lib().moonlab_mi355x_hea_tfim_energies(
+            self._h, arr, len(instances), precision, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            ctypes.byref(st)), "hea_tfim_energies")
+        self.last_stats = self._stats(st)
+        return e

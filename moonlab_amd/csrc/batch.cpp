@@ -1,0 +1,572 @@
+// batch.cpp -- host planner and C ABI of the persistent batch engine.
+//
+// Planning per circuit (independent of the rest of the batch, so a
+// circuit's results are the same alone and in any mix):
+//   1. single-qubit gates fold into one FP64 2x2 per qubit until a
+//      multi-qubit gate touches that qubit (the reference's run-length 1q
+//      fusion, src/optimization/fusion/fusion.c:247-292); a folded diagonal
+//      becomes a DIAG op, anything else a MAT1;
+//   2. every gate lowers to LDS ops: controlled gates to MAT1 with control
+//      masks, CZ / CPHASE / CRZ to DIAG, SWAP / Fredkin to SWAP;
+//   3. above the tile size, ops are packed into cache-blocked passes: a
+//      pass's tile set holds at most `tile_bits` qubits and must contain the
+//      target(s) of each non-diagonal op; controls and diagonal ops may sit
+//      outside it (they read fixed bits of the tile's base index).
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <climits>
+#include <complex>
+#include <vector>
+
+#include "../../include/moonlab_mi355x.h"
+#include "batch_device.h"
+#include "fusion.h"
+
+namespace mlx {
+
+hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
+                        const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
+                        uint64_t tile_slot, const double *draws, double *obs, uint64_t *outcomes, int grid,
+                        hipStream_t st);
+
+namespace {
+
+// An op before tile assignment: global qubit indices.
+struct GOp {
+    int      kind;
+    int      t1, t2;        // targets (MAT1: t1; SWAP: t1, t2); DIAG: qa = t1, qb = t2 (-1 = none)
+    uint64_t ctrl;          // control qubits (global bits)
+    double   m[8];
+};
+
+bool is_identity(const cd u[4])
+{
+    return u[0] == cd(1.0) && u[1] == cd(0.0) && u[2] == cd(0.0) && u[3] == cd(1.0);
+}
+
+void emit_1q(std::vector<GOp> &out, int q, const cd u[4])
+{
+    if (is_identity(u)) return;
+    GOp o{};
+    if (u[1] == cd(0.0) && u[2] == cd(0.0)) {
+        o.kind = BOP_DIAG;
+        o.t1 = q;
+        o.t2 = -1;
+        o.m[0] = u[0].real(); o.m[1] = u[0].imag();
+        o.m[2] = u[3].real(); o.m[3] = u[3].imag();
+        o.m[4] = 1.0; o.m[6] = 1.0;
+    } else {
+        o.kind = BOP_MAT1;
+        o.t1 = q;
+        o.t2 = -1;
+        for (int i = 0; i < 4; i++) {
+            o.m[2 * i] = u[i].real();
+            o.m[2 * i + 1] = u[i].imag();
+        }
+    }
+    out.push_back(o);
+}
+
+void mat_op(std::vector<GOp> &out, int t, uint64_t ctrl, cd u00, cd u01, cd u10, cd u11)
+{
+    GOp o{};
+    o.kind = BOP_MAT1;
+    o.t1 = t;
+    o.t2 = -1;
+    o.ctrl = ctrl;
+    const cd u[4] = {u00, u01, u10, u11};
+    for (int i = 0; i < 4; i++) {
+        o.m[2 * i] = u[i].real();
+        o.m[2 * i + 1] = u[i].imag();
+    }
+    out.push_back(o);
+}
+
+void diag2_op(std::vector<GOp> &out, int qa, int qb, uint64_t ctrl, cd d0, cd d1, cd d2, cd d3)
+{
+    GOp o{};
+    o.kind = BOP_DIAG;
+    o.t1 = qa;
+    o.t2 = qb;
+    o.ctrl = ctrl;
+    const cd d[4] = {d0, d1, d2, d3};
+    for (int i = 0; i < 4; i++) {
+        o.m[2 * i] = d[i].real();
+        o.m[2 * i + 1] = d[i].imag();
+    }
+    out.push_back(o);
+}
+
+// Steps 1 and 2: gate list -> global ops.
+bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out)
+{
+    std::vector<std::array<cd, 4>> pend((size_t)n);
+    std::vector<bool> has((size_t)n, false);
+    auto flush = [&](int q) {
+        if (has[(size_t)q]) emit_1q(out, q, pend[(size_t)q].data());
+        has[(size_t)q] = false;
+    };
+    const cd I(0.0, 1.0);
+    for (int i = 0; i < ng; i++) {
+        const moonlab_hip_gate_t &x = g[i];
+        if (x.type < 0 || x.type > MOONLAB_HIP_GATE_FREDKIN) return false;
+        int qs[3];
+        const int nq = gate_qubits(x, qs);
+        for (int j = 0; j < nq; j++)
+            if (qs[j] < 0 || qs[j] >= n || (j && qs[j] == qs[j - 1])) return false;
+        cd u[4];
+        if (gate_matrix_1q(x, u)) {
+            const int q = x.target;
+            if (!has[(size_t)q]) {
+                pend[(size_t)q] = {u[0], u[1], u[2], u[3]};
+                has[(size_t)q] = true;
+            } else {   // new = u * pending
+                const auto &p = pend[(size_t)q];
+                pend[(size_t)q] = {u[0] * p[0] + u[1] * p[2], u[0] * p[1] + u[1] * p[3],
+                                   u[2] * p[0] + u[3] * p[2], u[2] * p[1] + u[3] * p[3]};
+            }
+            continue;
+        }
+        for (int j = 0; j < nq; j++) flush(qs[j]);
+        const int c = x.control, t = x.target, e = x.extra;
+        const double co = cos(x.theta / 2.0), si = sin(x.theta / 2.0);
+        switch (x.type) {
+        case MOONLAB_HIP_GATE_CNOT: mat_op(out, t, 1ull << c, 0, 1, 1, 0); break;
+        case MOONLAB_HIP_GATE_CY: mat_op(out, t, 1ull << c, 0, I, -I, 0); break;
+        case MOONLAB_HIP_GATE_CY_TRUE: mat_op(out, t, 1ull << c, 0, -I, I, 0); break;
+        case MOONLAB_HIP_GATE_CRX: mat_op(out, t, 1ull << c, co, cd(0, -si), cd(0, -si), co); break;
+        case MOONLAB_HIP_GATE_CRY: mat_op(out, t, 1ull << c, co, -si, si, co); break;
+        case MOONLAB_HIP_GATE_TOFFOLI: mat_op(out, t, (1ull << c) | (1ull << e), 0, 1, 1, 0); break;
+        case MOONLAB_HIP_GATE_CZ: diag2_op(out, c, t, 0, 1, 1, 1, -1); break;
+        case MOONLAB_HIP_GATE_CPHASE: diag2_op(out, c, t, 0, 1, 1, 1, cd(cos(x.theta), sin(x.theta))); break;
+        case MOONLAB_HIP_GATE_CRZ:   // index = bit(c) << 1 | bit(t)
+            diag2_op(out, t, c, 0, 1, 1, cd(co, -si), cd(co, si));
+            break;
+        case MOONLAB_HIP_GATE_SWAP:
+        case MOONLAB_HIP_GATE_FREDKIN: {
+            GOp o{};
+            o.kind = BOP_SWAP;
+            o.t1 = x.type == MOONLAB_HIP_GATE_SWAP ? c : t;
+            o.t2 = x.type == MOONLAB_HIP_GATE_SWAP ? t : e;
+            o.ctrl = x.type == MOONLAB_HIP_GATE_FREDKIN ? 1ull << c : 0;
+            out.push_back(o);
+            break;
+        }
+        default: return false;
+        }
+    }
+    for (int q = 0; q < n; q++) flush(q);
+    return true;
+}
+
+uint64_t needs_local(const GOp &o)
+{
+    if (o.kind == BOP_MAT1) return 1ull << o.t1;
+    if (o.kind == BOP_SWAP) return (1ull << o.t1) | (1ull << o.t2);
+    return 0;
+}
+
+// Step 3: pack ops into passes and convert qubits to tile-local indices.
+void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass> &passes,
+                 std::vector<BOp> &ops)
+{
+    const bool resident = n <= tb;
+    const int anchor = resident ? n : 4;   // low qubits kept in every tile: 256-byte runs at complex128
+    uint64_t anchor_mask = anchor >= 64 ? ~0ull : ((1ull << anchor) - 1);
+    std::vector<const GOp *> cur;
+    uint64_t S = anchor_mask;
+    auto close = [&]() {
+        if (cur.empty()) return;
+        // fill the tile set to exactly tb qubits with the lowest unused ones
+        for (int q = 0; q < n && __builtin_popcountll(S) < (resident ? n : tb); q++) S |= 1ull << q;
+        BPass P{};
+        P.first_op = (int)ops.size();
+        P.num_ops = (int)cur.size();
+        P.nbits = 0;
+        int local[64];
+        for (int q = 0; q < 64; q++) local[q] = -1;
+        for (int q = 0; q < n; q++)
+            if (S & (1ull << q)) {
+                local[q] = P.nbits;
+                P.qubits[P.nbits++] = (int8_t)q;
+            }
+        for (const GOp *o : cur) {
+            BOp b{};
+            b.kind = o->kind;
+            for (int q = 0; q < n; q++) {
+                if (!(o->ctrl & (1ull << q))) continue;
+                if (local[q] >= 0) b.cl |= 1u << local[q];
+                else b.cg |= 1ull << q;
+            }
+            if (o->kind == BOP_DIAG) {
+                b.lq = local[o->t1] >= 0 ? local[o->t1] : -1 - o->t1;
+                b.lq2 = o->t2 < 0 ? INT32_MIN : (local[o->t2] >= 0 ? local[o->t2] : -1 - o->t2);
+            } else {
+                b.lq = local[o->t1];
+                b.lq2 = o->kind == BOP_SWAP ? local[o->t2] : INT32_MIN;
+            }
+            memcpy(b.m, o->m, sizeof b.m);
+            ops.push_back(b);
+        }
+        passes.push_back(P);
+        cur.clear();
+        S = anchor_mask;
+    };
+    for (const GOp &o : gops) {
+        const uint64_t need = needs_local(o);
+        if ((S | need) != S) {
+            if (__builtin_popcountll(S | need) > (resident ? n : tb)) close();
+            S |= need;
+        }
+        cur.push_back(&o);
+    }
+    close();
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T> size_t align_up(size_t x) { return (x + alignof(T) - 1) / alignof(T) * alignof(T); }
+
+}  // namespace
+}  // namespace mlx
+
+using namespace mlx;
+
+struct moonlab_mi355x_batch {
+    hipStream_t stream = nullptr;
+    int         wg_per_cu = 2;
+    int         tile_bits = kBatchMaxTileBits;
+    int         num_cus = 0;
+    // one device arena for the program (insts, order, passes, ops, draws) and one for results
+    void       *d_prog = nullptr;
+    size_t      prog_cap = 0;
+    void       *h_prog = nullptr;   // pinned staging
+    size_t      h_prog_cap = 0;
+    void       *d_res = nullptr;
+    size_t      res_cap = 0;
+    void       *h_res = nullptr;
+    size_t      h_res_cap = 0;
+    int        *d_counter = nullptr;
+    void       *d_scratch = nullptr;
+    size_t      scratch_cap = 0;
+    hipEvent_t  e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+int grow_dev(void **p, size_t *cap, size_t need)
+{
+    if (need <= *cap) return 0;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t sz = need + need / 4 + 4096;
+    if (hipMalloc(p, sz) != hipSuccess) return -1;
+    *cap = sz;
+    return 0;
+}
+
+int grow_host(void **p, size_t *cap, size_t need)
+{
+    if (need <= *cap) return 0;
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t sz = need + need / 4 + 4096;
+    if (hipHostMalloc(p, sz, hipHostMallocDefault) != hipSuccess) return -1;
+    *cap = sz;
+    return 0;
+}
+
+int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int count,
+              moonlab_mi355x_batch_stats_t *stats)
+{
+    const double t0 = now_s();
+    if (!b || count < 0 || (count > 0 && !items)) return MOONLAB_SCHED_BAD_ARG;
+    if (count == 0) return MOONLAB_SCHED_OK;
+    const int tbmax = b->tile_bits;
+    std::vector<BInst> insts((size_t)count);
+    std::vector<BPass> passes;
+    std::vector<BOp> ops;
+    std::vector<double> draws;
+    std::vector<std::pair<double, int>> cost((size_t)count);
+    int64_t obs_total = 0, out_total = 0;
+    int max_big = 0;
+    std::vector<GOp> gops;
+    for (int i = 0; i < count; i++) {
+        const moonlab_mi355x_batch_item_t &it = items[i];
+        const int n = it.num_qubits;
+        if (n < 1 || n > kBatchMaxQubits || (it.num_gates > 0 && !it.gates) || it.num_gates < 0 ||
+            it.num_shots < 0 || (it.num_shots > 0 && (!it.draws || !it.outcomes)) ||
+            (it.precision != MOONLAB_HIP_C128 && it.precision != MOONLAB_HIP_C64))
+            return MOONLAB_SCHED_BAD_ARG;
+        gops.clear();
+        if (!lower(it.gates, it.num_gates, n, gops)) return MOONLAB_SCHED_BAD_ARG;
+        BInst &I = insts[(size_t)i];
+        I.n = n;
+        I.prec = it.precision;
+        I.tile_bits = n < tbmax ? n : tbmax;
+        I.first_pass = (int)passes.size();
+        plan_passes(gops, n, I.tile_bits, passes, ops);
+        I.num_passes = (int)passes.size() - I.first_pass;
+        I.flags = 0;
+        if (it.observables & MOONLAB_MI355X_OBS_Z) I.flags |= BF_Z;
+        if (it.observables & MOONLAB_MI355X_OBS_ZZ) I.flags |= BF_ZZ;
+        if (it.observables & MOONLAB_MI355X_OBS_X) I.flags |= BF_X;
+        I.num_shots = it.num_shots;
+        I.obs_offset = obs_total;
+        obs_total += obs_size(n);
+        I.draw_offset = (int64_t)draws.size();
+        for (int s = 0; s < it.num_shots; s++) draws.push_back(it.draws[s]);
+        I.out_offset = out_total;
+        out_total += it.num_shots;
+        if (n > I.tile_bits && n > max_big) max_big = n;
+        // longest first: sweeps x amplitudes (resident circuits: ops x tile)
+        const double sweeps = n > I.tile_bits ? (double)(I.num_passes + 2) * 4.0 : 0.2 * (double)gops.size();
+        cost[(size_t)i] = {-(sweeps * (double)(1ull << n)), i};
+    }
+    std::stable_sort(cost.begin(), cost.end());
+    std::vector<int32_t> order((size_t)count);
+    for (int i = 0; i < count; i++) order[(size_t)i] = cost[(size_t)i].second;
+
+    // grid: as many workgroups as fit, bounded by the HBM slots large circuits need
+    int grid = b->num_cus * b->wg_per_cu;
+    if (grid > count) grid = count;
+    uint64_t slot_amps = 0, tile_slot = 0;
+    if (max_big) {
+        slot_amps = 1ull << max_big;
+        tile_slot = 2ull << (max_big - (tbmax < max_big ? tbmax : max_big));
+        const uint64_t per_wg = slot_amps * 16 + tile_slot * 8;
+        const uint64_t budget = 16ull << 30;
+        const uint64_t fit = budget / per_wg;
+        if ((uint64_t)grid > fit) grid = (int)(fit ? fit : 1);
+    } else {
+        slot_amps = 1;
+        tile_slot = 2;
+    }
+    const size_t scratch_bytes = (size_t)grid * (slot_amps * 16 + tile_slot * 8) + 256;
+    if (grow_dev(&b->d_scratch, &b->scratch_cap, scratch_bytes)) return MOONLAB_SCHED_OOM;
+
+    // program arena layout
+    size_t off = 0;
+    auto place = [&](size_t bytes) { const size_t o = off; off = (off + bytes + 255) / 256 * 256; return o; };
+    const size_t o_inst = place(insts.size() * sizeof(BInst));
+    const size_t o_order = place(order.size() * sizeof(int32_t));
+    const size_t o_pass = place((passes.size() + 1) * sizeof(BPass));
+    const size_t o_ops = place((ops.size() + 1) * sizeof(BOp));
+    const size_t o_draw = place((draws.size() + 1) * sizeof(double));
+    const size_t prog_bytes = off;
+    off = 0;
+    const size_t r_obs = place((size_t)(obs_total + 1) * sizeof(double));
+    const size_t r_out = place((size_t)(out_total + 1) * sizeof(uint64_t));
+    const size_t res_bytes = off;
+    if (grow_dev(&b->d_prog, &b->prog_cap, prog_bytes) || grow_host(&b->h_prog, &b->h_prog_cap, prog_bytes) ||
+        grow_dev(&b->d_res, &b->res_cap, res_bytes) || grow_host(&b->h_res, &b->h_res_cap, res_bytes))
+        return MOONLAB_SCHED_OOM;
+    char *hp = (char *)b->h_prog;
+    memcpy(hp + o_inst, insts.data(), insts.size() * sizeof(BInst));
+    memcpy(hp + o_order, order.data(), order.size() * sizeof(int32_t));
+    if (!passes.empty()) memcpy(hp + o_pass, passes.data(), passes.size() * sizeof(BPass));
+    if (!ops.empty()) memcpy(hp + o_ops, ops.data(), ops.size() * sizeof(BOp));
+    if (!draws.empty()) memcpy(hp + o_draw, draws.data(), draws.size() * sizeof(double));
+    char *dp = (char *)b->d_prog;
+    char *dr = (char *)b->d_res;
+    double2 *scratch = (double2 *)b->d_scratch;
+    double *tile_scratch = (double *)((char *)b->d_scratch + (size_t)grid * slot_amps * 16);
+
+    hipError_t e = hipMemcpyAsync(dp, hp, prog_bytes, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(b->e0, b->stream);
+    if (e == hipSuccess)
+        e = launch_batch((const BInst *)(dp + o_inst), (const int32_t *)(dp + o_order), count, b->d_counter,
+                         (const BPass *)(dp + o_pass), (const BOp *)(dp + o_ops), scratch, slot_amps, tile_scratch,
+                         tile_slot, (const double *)(dp + o_draw), (double *)(dr + r_obs),
+                         (uint64_t *)(dr + r_out), grid, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(b->e1, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->h_res, dr, res_bytes, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        fprintf(stderr, "moonlab-mi355x batch: %s\n", hipGetErrorString(e));
+        return MOONLAB_SCHED_INTERNAL;
+    }
+    const double *obs = (const double *)((char *)b->h_res + r_obs);
+    const uint64_t *outc = (const uint64_t *)((char *)b->h_res + r_out);
+    for (int i = 0; i < count; i++) {
+        moonlab_mi355x_batch_item_t &it = items[i];
+        const BInst &I = insts[(size_t)i];
+        const int n = I.n;
+        const double *o = obs + I.obs_offset;
+        if (it.norm) *it.norm = o[0];
+        if (it.expect_z)
+            for (int q = 0; q < n; q++) it.expect_z[q] = 1.0 - 2.0 * o[1 + q];
+        if (it.zz_chain)
+            for (int q = 0; q + 1 < n; q++) it.zz_chain[q] = o[1 + n + q];
+        if (it.expect_x)
+            for (int q = 0; q < n; q++) it.expect_x[q] = o[1 + n + (n > 1 ? n - 1 : 0) + q];
+        if (it.num_shots > 0) memcpy(it.outcomes, outc + I.out_offset, (size_t)it.num_shots * sizeof(uint64_t));
+    }
+    if (stats) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, b->e0, b->e1);
+        stats->device_seconds = ms * 1e-3;
+        stats->wall_seconds = now_s() - t0;
+        stats->workgroups = grid;
+        stats->tile_bits = tbmax;
+        int big_passes = 0;
+        for (const BInst &I : insts)
+            if (I.n > I.tile_bits) big_passes += I.num_passes;
+        stats->total_passes = big_passes;
+        stats->total_ops = (int)ops.size();
+    }
+    return MOONLAB_SCHED_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+moonlab_mi355x_batch_t *moonlab_mi355x_batch_create(void)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        fprintf(stderr, "moonlab-mi355x batch: no HIP device visible\n");
+        return nullptr;
+    }
+    moonlab_mi355x_batch_t *b = new (std::nothrow) moonlab_mi355x_batch_t();
+    if (!b) return nullptr;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (b->num_cus <= 0) b->num_cus = 256;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&b->d_counter, 256) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+        hipEventCreate(&b->e1) != hipSuccess) {
+        moonlab_mi355x_batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void moonlab_mi355x_batch_free(moonlab_mi355x_batch_t *b)
+{
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    hipFree(b->d_prog);
+    hipFree(b->d_res);
+    hipFree(b->d_counter);
+    hipFree(b->d_scratch);
+    if (b->h_prog) hipHostFree(b->h_prog);
+    if (b->h_res) hipHostFree(b->h_res);
+    if (b->e0) hipEventDestroy(b->e0);
+    if (b->e1) hipEventDestroy(b->e1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int moonlab_mi355x_batch_set_config(moonlab_mi355x_batch_t *b, int wg_per_cu, int tile_bits)
+{
+    if (!b) return MOONLAB_SCHED_BAD_ARG;
+    if (wg_per_cu) {
+        if (wg_per_cu < 1 || wg_per_cu > 2) return MOONLAB_SCHED_BAD_ARG;
+        b->wg_per_cu = wg_per_cu;
+    }
+    if (tile_bits) {
+        if (tile_bits < 8 || tile_bits > kBatchMaxTileBits) return MOONLAB_SCHED_BAD_ARG;
+        b->tile_bits = tile_bits;
+    }
+    return MOONLAB_SCHED_OK;
+}
+
+int moonlab_mi355x_batch_run(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int count,
+                             moonlab_mi355x_batch_stats_t *stats)
+{
+    return run_batch(b, items, count, stats);
+}
+
+int moonlab_mi355x_batch_autotune(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int count,
+                                  moonlab_mi355x_batch_stats_t *stats)
+{
+    if (!b) return MOONLAB_SCHED_BAD_ARG;
+    double best = 1e300;
+    int best_wg = b->wg_per_cu, best_tb = b->tile_bits;
+    for (int wg = 1; wg <= 2; wg++)
+        for (int tb = 10; tb <= kBatchMaxTileBits; tb++) {
+            b->wg_per_cu = wg;
+            b->tile_bits = tb;
+            moonlab_mi355x_batch_stats_t st{};
+            const int rc = run_batch(b, items, count, &st);
+            if (rc) return rc;
+            if (st.device_seconds < best) {
+                best = st.device_seconds;
+                best_wg = wg;
+                best_tb = tb;
+            }
+        }
+    b->wg_per_cu = best_wg;
+    b->tile_bits = best_tb;
+    return run_batch(b, items, count, stats);
+}
+
+int moonlab_mi355x_hea_tfim_energies(moonlab_mi355x_batch_t *b, const moonlab_mi355x_hea_t *inst, int count,
+                                     int precision, double *energies, moonlab_mi355x_batch_stats_t *stats)
+{
+    if (!b || count < 0 || (count > 0 && (!inst || !energies))) return MOONLAB_SCHED_BAD_ARG;
+    std::vector<std::vector<moonlab_hip_gate_t>> gl((size_t)count);
+    std::vector<moonlab_mi355x_batch_item_t> items((size_t)count);
+    std::vector<double> zz, xx;
+    size_t zoff = 0, xoff = 0;
+    for (int i = 0; i < count; i++) {
+        zoff += (size_t)(inst[i].num_qubits > 1 ? inst[i].num_qubits - 1 : 1);
+        xoff += (size_t)inst[i].num_qubits;
+    }
+    zz.assign(zoff, 0.0);
+    xx.assign(xoff, 0.0);
+    zoff = xoff = 0;
+    for (int i = 0; i < count; i++) {
+        const int n = inst[i].num_qubits, L = inst[i].num_layers;
+        if (n < 1 || n > kBatchMaxQubits || L < 0 || (L > 0 && !inst[i].angles)) return MOONLAB_SCHED_BAD_ARG;
+        auto &g = gl[(size_t)i];
+        for (int l = 0; l < L; l++) {
+            for (int q = 0; q < n; q++) {
+                const double *a = inst[i].angles + ((size_t)l * n + q) * 2;
+                g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_RY, q, -1, -1, a[0], 0, 0});
+                g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_RZ, q, -1, -1, a[1], 0, 0});
+            }
+            for (int q = 0; q + 1 < n; q++)
+                g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_CZ, q + 1, q, -1, 0, 0, 0});
+        }
+        moonlab_mi355x_batch_item_t &it = items[(size_t)i];
+        memset(&it, 0, sizeof it);
+        it.num_qubits = n;
+        it.precision = precision;
+        it.gates = g.data();
+        it.num_gates = (int)g.size();
+        it.observables = MOONLAB_MI355X_OBS_ZZ | MOONLAB_MI355X_OBS_X;
+        it.zz_chain = zz.data() + zoff;
+        it.expect_x = xx.data() + xoff;
+        zoff += (size_t)(n > 1 ? n - 1 : 1);
+        xoff += (size_t)n;
+    }
+    const int rc = run_batch(b, items.data(), count, stats);
+    if (rc) return rc;
+    // E = -sum_q <Z_q Z_q+1> - h sum_q <X_q>, summed in qubit order
+    for (int i = 0; i < count; i++) {
+        const int n = inst[i].num_qubits;
+        double e = 0.0;
+        for (int q = 0; q + 1 < n; q++) e -= items[(size_t)i].zz_chain[q];
+        double sx = 0.0;
+        for (int q = 0; q < n; q++) sx += items[(size_t)i].expect_x[q];
+        energies[i] = e - inst[i].h * sx;
+    }
+    return MOONLAB_SCHED_OK;
+}
+
+}  // extern "C"

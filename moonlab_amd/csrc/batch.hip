@@ -1,0 +1,465 @@
+// batch.hip -- one persistent launch that simulates thousands of small,
+// uneven circuits.
+//
+// Each 1024-thread workgroup takes the next circuit from an atomic work queue
+// (the host orders it longest-first) and runs it start to finish:
+//   * n <= tile bits (12): the whole state lives in LDS (64 KiB at
+//     complex128); gates are LDS sweeps; observables and shots are computed
+//     from LDS.  HBM sees only the program and the results.
+//   * n > 12: the state lives in the workgroup's HBM slot and every pass
+//     streams it through LDS in 2^12-amplitude tiles (cache blocking): the
+//     host planner packs each run of gates whose non-diagonal qubits fit in a
+//     12-qubit tile set into one pass, so a circuit costs a few HBM sweeps
+//     per layer instead of one per gate.
+// Workgroups never wait on each other; every workgroup exits when the queue
+// is empty, so the grid always drains.
+//
+// Reference counterparts: the CPU VQE loop (src/algorithms/vqe.c:764-781,
+// 1287-1350, 2156-2257) and the one-block-per-search batch kernel
+// (src/optimization/gpu/backends/gpu_cuda.cu:350).
+#include <hip/hip_runtime.h>
+
+#include <limits.h>
+
+#include "batch_device.h"
+#include "sv_device.h"
+
+namespace mlx {
+namespace {
+
+constexpr int NT = kBatchThreads;
+constexpr int NW = NT / 64;
+constexpr int kMaxObs = 1 + kBatchMaxQubits + (kBatchMaxQubits - 1) + kBatchMaxQubits;
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+struct Shared {
+    double2  tile[1 << kBatchMaxTileBits];   // 64 KiB: state tile, later the CDF
+    uint64_t dlo[64], dhi[64];               // deposit tables of the current tile set
+    double   red[NW][kMaxObs];               // per-wave partials
+    double   acc[kMaxObs];                   // per-circuit accumulators
+    double   res[8];                         // block_reduce results
+    int      next;
+    int      flag;
+    int      last_nz;
+};
+
+// Block reduction of v[0..nv): wave butterflies, then waves summed in order.
+// Result j lands in out[j] (for j < nv) on every thread after the call.
+__device__ void block_reduce(Shared &sh, double *v, int nv, double *out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j < nv; j++) {
+        const double s = wsum(v[j]);
+        if (lane == 0) sh.red[wave][j] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv) {
+        double s = 0.0;
+        for (int w = 0; w < NW; w++) s += sh.red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+template <typename R>
+__device__ __forceinline__ double p2(const cplx_t<R> &x)
+{
+    const double r = (double)x.x, i = (double)x.y;
+    return fmar(r, r, i * i);
+}
+
+__device__ __forceinline__ int bit_of(int32_t code, int l, uint64_t base)
+{
+    // code >= 0: local tile bit; code < 0 (and not INT32_MIN): global qubit -1 - code
+    if (code == INT32_MIN) return 0;
+    return code >= 0 ? (l >> code) & 1 : (int)((base >> (-1 - code)) & 1ull);
+}
+
+template <typename R>
+__device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
+{
+    if ((base & op.cg) != op.cg) return;          // a global control is 0 for this whole tile
+    const int tid = threadIdx.x;
+    if (op.kind == BOP_MAT1) {
+        const R m0 = (R)op.m[0], m1 = (R)op.m[1], m2 = (R)op.m[2], m3 = (R)op.m[3];
+        const R m4 = (R)op.m[4], m5 = (R)op.m[5], m6 = (R)op.m[6], m7 = (R)op.m[7];
+        const int half = T >> 1;
+        for (int p = tid; p < half; p += NT) {
+            const int i0 = (int)ins0((uint64_t)p, op.lq), i1 = i0 | (1 << op.lq);
+            if ((i0 & (int)op.cl) != (int)op.cl) continue;
+            const cplx_t<R> a = t[i0], b = t[i1];
+            cplx_t<R> x, y;
+            x.x = fmar(m0, a.x, -(m1 * a.y)) + fmar(m2, b.x, -(m3 * b.y));
+            x.y = fmar(m0, a.y, m1 * a.x) + fmar(m2, b.y, m3 * b.x);
+            y.x = fmar(m4, a.x, -(m5 * a.y)) + fmar(m6, b.x, -(m7 * b.y));
+            y.y = fmar(m4, a.y, m5 * a.x) + fmar(m6, b.y, m7 * b.x);
+            t[i0] = x;
+            t[i1] = y;
+        }
+    } else if (op.kind == BOP_DIAG) {
+        for (int l = tid; l < T; l += NT) {
+            if ((l & (int)op.cl) != (int)op.cl) continue;
+            const int k = (bit_of(op.lq2, l, base) << 1) | bit_of(op.lq, l, base);
+            t[l] = cmul(t[l], (R)op.m[2 * k], (R)op.m[2 * k + 1]);
+        }
+    } else {   // BOP_SWAP
+        const int b1 = 1 << op.lq, b2 = 1 << op.lq2;
+        for (int l = tid; l < T; l += NT) {
+            if ((l & b1) && !(l & b2) && (l & (int)op.cl) == (int)op.cl) {
+                const int j = (l ^ b1) | b2;
+                const cplx_t<R> x = t[l];
+                t[l] = t[j];
+                t[j] = x;
+            }
+        }
+    }
+}
+
+__device__ void build_deposit(Shared &sh, const BPass &p)
+{
+    const int tid = threadIdx.x;
+    if (tid < 128) {
+        const int j = tid & 63, hi = tid >> 6;
+        uint64_t o = 0;
+        for (int b = 0; b < 6; b++) {
+            const int qi = hi * 6 + b;
+            if (qi < p.nbits && ((j >> b) & 1)) o |= 1ull << p.qubits[qi];
+        }
+        if (hi) sh.dhi[j] = o;
+        else sh.dlo[j] = o;
+    }
+}
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const BPass &p)
+{
+    for (int b = 0; b < p.nbits; b++) t = ins0(t, p.qubits[b]);
+    return t;
+}
+
+// Observables of one tile, added into sh.acc (value layout of obs_size()).
+// Measurement pass 0 uses the low `tb` qubits as the tile (tiles in index
+// order: norm, P(1), ZZ and X on the low qubits); pass 1 uses the top `tb`
+// qubits (X on the qubits pass 0 could not pair).  Values are reduced eight
+// at a time to bound register use; the order of every sum is fixed.
+template <typename R>
+__device__ void measure_tile(Shared &sh, const cplx_t<R> *t, int T, int tb, int n, uint64_t base,
+                             int flags, int mpass, double *tile_total)
+{
+    const int tid = threadIdx.x;
+    const int nzz = n > 1 ? n - 1 : 0;
+    const int xb = 1 + n + nzz;               // first X slot
+    const int nv = xb + n;
+    const int qoff = mpass == 0 ? 0 : n - tb;  // global qubit of local bit 0
+    const int half = T >> 1;
+    for (int c0 = 0; c0 < nv; c0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = 0.0;
+        bool any = false;
+        for (int j = 0; j < 8 && c0 + j < nv; j++) {
+            const int vi = c0 + j;
+            if (vi == 0) any = any || mpass == 0;
+            else if (vi < 1 + n) any = any || (mpass == 0 && (flags & BF_Z));
+            else if (vi < xb) any = any || (mpass == 0 && (flags & BF_ZZ));
+            else any = any || (flags & BF_X);
+        }
+        if (!any) continue;   // uniform across the block
+        if (mpass == 0 && c0 < xb) {
+            for (int l = tid; l < T; l += NT) {
+                const double p = p2<R>(t[l]);
+                const uint64_t full = base | (uint64_t)l;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int vi = c0 + j;
+                    if (vi == 0) {
+                        v[j] += p;
+                    } else if (vi < 1 + n) {
+                        if ((flags & BF_Z) && ((full >> (vi - 1)) & 1ull)) v[j] += p;
+                    } else if (vi < xb) {
+                        const int q = vi - 1 - n;
+                        if (flags & BF_ZZ) v[j] += (((full >> q) ^ (full >> (q + 1))) & 1ull) ? -p : p;
+                    }
+                }
+            }
+        }
+        if (flags & BF_X) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int vi = c0 + j;
+                if (vi < xb || vi >= nv) continue;
+                const int q = vi - xb, lb = q - qoff;
+                if (lb < 0 || lb >= tb || (mpass == 1 && q < tb)) continue;
+                double s2 = 0.0;
+                for (int pp = tid; pp < half; pp += NT) {
+                    const int i0 = (int)ins0((uint64_t)pp, lb), i1 = i0 | (1 << lb);
+                    s2 += fmar((double)t[i0].x, (double)t[i1].x, (double)t[i0].y * (double)t[i1].y);
+                }
+                v[j] += s2 + s2;
+            }
+        }
+        const int cnt = nv - c0 < 8 ? nv - c0 : 8;
+        block_reduce(sh, v, cnt, sh.res);
+        if (tid < cnt) sh.acc[c0 + tid] += sh.res[tid];
+        if (c0 == 0 && tile_total && tid == 0) *tile_total = sh.res[0];
+        __syncthreads();
+    }
+}
+
+// Inclusive scan of p over T entries in LDS (as doubles), index order.
+__device__ void lds_scan(Shared &sh, double *cdf, int T, double offset)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = T >= NT ? T / NT : 1;
+    const int lo = tid * per;
+    double s = 0.0;
+    if (lo < T)
+        for (int e = 0; e < per; e++) s += cdf[lo + e];
+    double x = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh.red[wave][0] = x;
+    __syncthreads();
+    double run = offset;
+    for (int w = 0; w < wave; w++) run += sh.red[w][0];
+    run += x - s;
+    __syncthreads();
+    if (lo < T)
+        for (int e = 0; e < per; e++) {
+            run += cdf[lo + e];
+            cdf[lo + e] = run;
+        }
+    __syncthreads();
+}
+
+template <typename R>
+__device__ void run_circuit(Shared &sh, const BInst &I, const BPass *passes, const BOp *ops,
+                            double2 *slot_raw, double *tile_sums, const double *draws, double *obs,
+                            uint64_t *outcomes)
+{
+    const int tid = threadIdx.x;
+    const int n = I.n, tb = I.tile_bits, T = 1 << tb;
+    cplx_t<R> *t = (cplx_t<R> *)sh.tile;
+    cplx_t<R> *slot = (cplx_t<R> *)slot_raw;
+    const uint64_t ntiles = 1ull << (n - tb);
+    const bool resident = n == tb;
+    for (int j = tid; j < kMaxObs; j += NT) sh.acc[j] = 0.0;
+
+    // ---- gate passes ----
+    for (int pi = 0; pi < I.num_passes; pi++) {
+        const BPass P = passes[I.first_pass + pi];
+        if (!resident) build_deposit(sh, P);
+        __syncthreads();
+        for (uint64_t ti = 0; ti < ntiles; ti++) {
+            const uint64_t base = resident ? 0 : tile_base(ti, P);
+            if (pi == 0) {
+                for (int l = tid; l < T; l += NT) t[l] = cplx_t<R>{(base | l) == 0 ? R(1) : R(0), R(0)};
+            } else if (!resident) {
+                for (int l = tid; l < T; l += NT) t[l] = slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]];
+            }
+            __syncthreads();
+            for (int k = 0; k < P.num_ops; k++) {
+                apply_op<R>(t, T, ops[P.first_op + k], base);
+                __syncthreads();
+            }
+            if (!resident) {
+                for (int l = tid; l < T; l += NT) slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]] = t[l];
+                __syncthreads();
+            }
+        }
+    }
+    if (I.num_passes == 0) {   // empty circuit: |0..0>
+        for (uint64_t ti = 0; ti < (resident ? 1 : ntiles); ti++) {
+            for (int l = tid; l < T; l += NT)
+                t[l] = cplx_t<R>{(ti == 0 && l == 0) ? R(1) : R(0), R(0)};
+            __syncthreads();
+            if (!resident)
+                for (int l = tid; l < T; l += NT) slot[(ti << tb) | l] = t[l];
+            __syncthreads();
+        }
+    }
+
+    // ---- measurement pass 0: tiles of the low tb qubits, index order ----
+    const bool want_shots = I.num_shots > 0;
+    for (uint64_t ti = 0; ti < ntiles; ti++) {
+        const uint64_t base = ti << tb;
+        if (!resident) {
+            for (int l = tid; l < T; l += NT) t[l] = slot[base | l];
+            __syncthreads();
+        }
+        double tt = 0.0;
+        measure_tile<R>(sh, t, T, tb, n, base, I.flags, 0, &tt);
+        if (!resident && tid == 0 && want_shots) tile_sums[ti] = tt;
+        __syncthreads();
+    }
+    // ---- measurement pass 1: X on the qubits above the low tile ----
+    if ((I.flags & BF_X) && !resident) {
+        BPass P;
+        P.nbits = tb;
+        for (int b = 0; b < tb; b++) P.qubits[b] = (int8_t)(n - tb + b);
+        build_deposit(sh, P);
+        __syncthreads();
+        for (uint64_t ti = 0; ti < ntiles; ti++) {
+            const uint64_t base = tile_base(ti, P);
+            for (int l = tid; l < T; l += NT) t[l] = slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]];
+            __syncthreads();
+            measure_tile<R>(sh, t, T, tb, n, base, BF_X, 1, nullptr);
+        }
+    }
+    const int nv = 1 + n + (n > 1 ? n - 1 : 0) + n;
+    for (int j = tid; j < nv; j += NT) obs[I.obs_offset + j] = sh.acc[j];
+
+    // ---- shots: first index with u < cdf_i, else the last nonzero ----
+    if (!want_shots) return;
+    double *cdf = (double *)sh.tile;
+    if (resident) {
+        double pv[(1 << kBatchMaxTileBits) / NT > 0 ? (1 << kBatchMaxTileBits) / NT : 1];
+        int k = 0;
+        for (int l = tid; l < T; l += NT) pv[k++] = p2<R>(t[l]);
+        if (tid == 0) sh.last_nz = -1;
+        __syncthreads();
+        k = 0;
+        int mylast = -1;
+        for (int l = tid; l < T; l += NT) {
+            cdf[l] = pv[k];
+            if (pv[k] > 0.0) mylast = l;
+            k++;
+        }
+        if (mylast >= 0) atomicMax(&sh.last_nz, mylast);
+        __syncthreads();
+        lds_scan(sh, cdf, T, 0.0);
+        const int fallback = sh.last_nz < 0 ? 0 : sh.last_nz;
+        for (int s = tid; s < I.num_shots; s += NT) {
+            const double u = draws[I.draw_offset + s];
+            int lo = 0, hi = T;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (u < cdf[mid]) hi = mid;
+                else lo = mid + 1;
+            }
+            outcomes[I.out_offset + s] = (uint64_t)(lo < T ? lo : fallback);
+        }
+        __syncthreads();
+        return;
+    }
+    // global state: tile prefix in place over tile_sums[ntiles .. 2 ntiles)
+    double *incl = tile_sums + ntiles;
+    if (tid == 0) {
+        double run = 0.0;
+        for (uint64_t i = 0; i < ntiles; i++) {
+            run += tile_sums[i];
+            incl[i] = run;
+        }
+    }
+    __syncthreads();
+    // each shot's starting tile
+    const int per_thread = (I.num_shots + NT - 1) / NT;
+    for (int rep = 0; rep < per_thread; rep++) {
+        const int s = tid + rep * NT;
+        uint64_t tile = ntiles;
+        double u = 2.0;
+        if (s < I.num_shots) {
+            u = draws[I.draw_offset + s];
+            uint64_t lo = 0, hi = ntiles;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (u < incl[mid]) hi = mid;
+                else lo = mid + 1;
+            }
+            tile = lo;
+        }
+        uint64_t result = UINT64_MAX;
+        // walk tiles in order; every thread joins every step so barriers match
+        for (uint64_t ti = 0; ti < ntiles; ti++) {
+            if (tid == 0) sh.flag = 0;
+            __syncthreads();
+            if (tile == ti && result == UINT64_MAX) sh.flag = 1;
+            __syncthreads();
+            const int need = sh.flag;
+            __syncthreads();   // every thread has read the flag before thread 0 clears it again
+            if (!need) continue;
+            for (int l = tid; l < T; l += NT) cdf[l] = p2<R>(slot[(ti << tb) | l]);
+            __syncthreads();
+            lds_scan(sh, cdf, T, ti ? incl[ti - 1] : 0.0);
+            if (tile == ti && result == UINT64_MAX) {
+                int lo = 0, hi = T;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (u < cdf[mid]) hi = mid;
+                    else lo = mid + 1;
+                }
+                if (lo < T) result = (ti << tb) | (uint64_t)lo;
+                else tile = ti + 1;   // rounding left u past this tile: continue in the next
+            }
+            __syncthreads();
+        }
+        if (s < I.num_shots && result == UINT64_MAX) {
+            // u >= total mass: the last index with nonzero probability
+            uint64_t last = 0;
+            for (uint64_t ti = ntiles; ti-- > 0;) {
+                if (tile_sums[ti] > 0.0) {
+                    for (int l = T - 1; l >= 0; l--)
+                        if (p2<R>(slot[(ti << tb) | l]) > 0.0) {
+                            last = (ti << tb) | (uint64_t)l;
+                            break;
+                        }
+                    break;
+                }
+            }
+            result = last;
+        }
+        if (s < I.num_shots) outcomes[I.out_offset + s] = result;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_batch(const BInst *__restrict__ insts,
+                                              const int32_t *__restrict__ order, int ninst,
+                                              int *__restrict__ counter, const BPass *__restrict__ passes,
+                                              const BOp *__restrict__ ops, double2 *__restrict__ scratch,
+                                              uint64_t slot_amps, double *__restrict__ tile_scratch,
+                                              uint64_t tile_slot, const double *__restrict__ draws,
+                                              double *__restrict__ obs, uint64_t *__restrict__ outcomes)
+{
+    __shared__ Shared sh;
+    double2 *slot = scratch + (uint64_t)blockIdx.x * slot_amps;
+    double *ts = tile_scratch + (uint64_t)blockIdx.x * tile_slot;
+    for (;;) {
+        if (threadIdx.x == 0) sh.next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int k = sh.next;
+        __syncthreads();
+        if (k >= ninst) break;
+        const BInst I = insts[order[k]];
+        if (I.prec == 1)
+            run_circuit<float>(sh, I, passes, ops, slot, ts, draws, obs, outcomes);
+        else
+            run_circuit<double>(sh, I, passes, ops, slot, ts, draws, obs, outcomes);
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_counter(int *c) { *c = 0; }
+
+}  // namespace
+
+hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
+                        const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
+                        uint64_t tile_slot, const double *draws, double *obs, uint64_t *outcomes, int grid,
+                        hipStream_t st)
+{
+    k_reset_counter<<<1, 1, 0, st>>>(counter);
+    k_batch<<<grid, NT, 0, st>>>(insts, order, ninst, counter, passes, ops, scratch, slot_amps, tile_scratch,
+                                 tile_slot, draws, obs, outcomes);
+    return hipGetLastError();
+}
+
+}  // namespace mlx

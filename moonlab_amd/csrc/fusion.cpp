@@ -116,6 +116,15 @@ void apply_local(const GateAction &a, const int *qs, int k, cd *v)
 
 }  // namespace
 
+bool gate_matrix_1q(const moonlab_hip_gate_t &g, cd u[4])
+{
+    int q[3];
+    if (gate_qubits(g, q) != 1) return false;
+    const GateAction a = action_of(g);
+    for (int i = 0; i < 4; i++) u[i] = a.u[i];
+    return true;
+}
+
 int gate_qubits(const moonlab_hip_gate_t &g, int q[3])
 {
     int n = 0;

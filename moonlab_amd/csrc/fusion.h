@@ -27,6 +27,9 @@ int gate_qubits(const moonlab_hip_gate_t &g, int q[3]);
 // gate_qubits), row-major 2^nq x 2^nq.
 void gate_dense(const moonlab_hip_gate_t &g, cd *m);
 
+// 2x2 matrix (row-major) of a single-qubit gate; false for multi-qubit codes.
+bool gate_matrix_1q(const moonlab_hip_gate_t &g, cd u[4]);
+
 // Greedy in-order fusion into blocks of at most kmax qubits, each composed
 // in FP64 (left-multiplication in gate order).
 std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kmax);
