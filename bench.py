@@ -73,6 +73,124 @@ def cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | None:
             "gates_per_s": round(done / dt, 3)}
 
 
+def batch_cpu_baseline(inst, budget_s: float) -> dict | None:
+    """The reference's C on the host: the VQE energy loop (gate_ry, gate_rz,
+    gate_cz, then measurement_correlation_zz / measurement_expectation_x) over
+    the batch in its own order until `budget_s` of CPU work."""
+    try:
+        from oracle.oracle import RefLib
+        from moonlab_amd.workloads import hea_gate_list
+        ref = RefLib()
+    except (FileNotFoundError, OSError) as e:
+        print(f"[bench] batch cpu baseline unavailable: {e}", file=sys.stderr, flush=True)
+        return None
+    done = 0
+    energies = []
+    t0 = time.perf_counter()
+    for n, L, ang, h in inst:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        s = ref.state(n, hea_gate_list(n, L, ang))
+        e = -sum(ref.L.measurement_correlation_zz(s, q, q + 1) for q in range(n - 1))
+        e -= h * sum(ref.L.measurement_expectation_x(s, q) for q in range(n))
+        ref.free(s)
+        energies.append(e)
+        done += 1
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(done / dt, 3), "unit": "circuits/s", "cores": cores, "kind": "reference",
+            "sample": f"first {done} of {len(inst)} instances in batch order ({dt:.1f} s), "
+                      "tsotchke/moonlab C gate_* + measurement_* built by oracle/Makefile",
+            "energies": energies}
+
+
+def run_batch_workload(m, count: int, steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dict:
+    from moonlab_amd import workloads
+    inst = workloads.hea_batch(count, SEED)
+    b = m.Batch()
+    circuits = [{"n": n, "gates": workloads.hea_gate_list(n, L, ang), "observables": m.OBS_ZZ | m.OBS_X}
+                for n, L, ang, h in inst]
+    t = time.perf_counter()
+    b.run(circuits, autotune=True)            # picks workgroups/CU and tile bits on this batch
+    tuned = dict(b.last_stats)
+    print(f"[bench] batch autotune {time.perf_counter() - t:.1f}s -> {tuned}", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        b.hea_tfim(inst)
+    walls, devs = [], []
+    e = None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        e = b.hea_tfim(inst)
+        walls.append(time.perf_counter() - t0)
+        devs.append(b.last_stats["device_seconds"])
+        print(f"[bench] batch step {k + 1}/{steps}: {walls[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    wall = float(np.sum(walls))
+    out = {"workload": f"{count} hardware-efficient-ansatz instances, n ~ U{{8..18}}, layers ~ U{{4..12}}, "
+                       f"RY+RZ per qubit + CZ ladder, TFIM energy (h ~ U[0.5,1.5]), complex128, seed {SEED}",
+           "circuits_per_s": round(count * steps / wall, 1),
+           "ms_per_batch": round(wall / steps * 1e3, 3),
+           "device_ms_per_batch": round(float(np.mean(devs)) * 1e3, 3),
+           "launches_per_batch": 1,
+           "autotune": {"tile_bits": tuned["tile_bits"], "workgroups": tuned["workgroups"]},
+           "hbm_passes_per_batch": tuned["total_passes"]}
+    if with_cpu:
+        cpu = batch_cpu_baseline(inst, cpu_seconds)
+        if cpu:
+            got = np.array(cpu.pop("energies"))
+            out["max_abs_energy_error_vs_reference"] = float(np.max(np.abs(e[:len(got)] - got))) if len(got) else None
+            out["cpu_baseline"] = cpu
+    return out
+
+
+def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
+    """Poisson arrivals submitted in real time; latency = finish - arrival."""
+    import threading
+    from moonlab_amd import workloads
+    reqs = workloads.serving_requests(count, SEED + 5, rate)
+    srv = m.Server(sequential=sequential)
+    try:
+        # warm the engines (state pool, code objects) outside the measured stream
+        for r in reqs[:4]:
+            srv.wait(srv.submit(r["n"] if r["n"] <= 16 else 12, r["gates"][:8] if r["n"] > 16 else r["gates"],
+                                shots=16, seed=1, precision=r["precision"]))
+        done = [None] * len(reqs)
+        tickets = [None] * len(reqs)
+        ready = threading.Semaphore(0)
+
+        def collect():
+            for i in range(len(reqs)):
+                ready.acquire()
+                done[i] = srv.wait(tickets[i])
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = srv.now() + 0.05
+        for i, r in enumerate(reqs):
+            delay = t0 + r["arrival"] - srv.now()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(r["n"], r["gates"], shots=r["shots"], seed=r["shot_seed"],
+                                    precision=r["precision"])
+            ready.release()
+            if (i + 1) % 100 == 0:
+                print(f"[bench] serving ({'plain' if sequential else 'served'}) submitted {i + 1}/{len(reqs)}",
+                      file=sys.stderr, flush=True)
+        th.join()
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = np.array([d.finish_time - d.submit_time for d in done])
+    first = min(d.submit_time for d in done)
+    last = max(d.finish_time for d in done)
+    bad = sum(1 for d in done if d.status != 0)
+    return {"circuits_per_s": round(len(done) / (last - first), 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "makespan_s": round(last - first, 3), "failed": bad,
+            "batches": int(st["batches"]), "largest_batch": int(st["largest_batch"]),
+            "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2 ** 30, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +200,12 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="short run for profiler counters: one device copy of the state "
                          "(known bytes, the calibration kernel) and the first layer of the sweep")
@@ -165,6 +289,18 @@ def main() -> int:
 
     value = bytes_step * K / wall / 1e12
     cpu = None if args.no_cpu else cpu_baseline(n, gates, args.cpu_seconds)
+    batch = None
+    if not args.no_batch:
+        batch = run_batch_workload(m, args.batch_count, args.batch_steps, 1, 15.0, not args.no_cpu)
+    serving = None
+    if not args.no_serve:
+        served = run_serving_workload(m, args.serve_count, args.serve_rate, False)
+        plain = run_serving_workload(m, args.serve_count, args.serve_rate, True)
+        serving = {"workload": f"{args.serve_count} Poisson requests at {args.serve_rate:g}/s: random-layer "
+                               "circuits, n in 10..30 with P(n) ~ 2^(-(n-10)/4), depth ~ U{10..50}, "
+                               "complex128/complex64 50/50, 1024 shots + <Z_q> each, 256 GiB HBM budget",
+                   "served": served, "plain_sequential": plain,
+                   "circuits_per_s_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2)}
     out = {
         "metric": "HBM TB/s on 30q gate sweep",
         "value": round(value, 4),
@@ -192,6 +328,8 @@ def main() -> int:
                      "traffic": traffic},
         "cpu_baseline": cpu,
         "norm_error": abs(nrm - 1.0),
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

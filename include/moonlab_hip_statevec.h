@@ -157,6 +157,14 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_fused(moonlab_hip_state
                                                                const moonlab_hip_gate_t *gates,
                                                                int num_gates, int kmax);
 
+/* Addition: apply a gate list cache-blocked: single-qubit runs fold into one
+ * FP64 2x2 per qubit, then ops are packed into passes whose non-diagonal
+ * targets fit a 12-qubit tile set; each pass streams the state through LDS
+ * in 4096-amplitude tiles once and applies all of its ops there. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *state,
+                                                                 const moonlab_hip_gate_t *gates,
+                                                                 int num_gates);
+
 /* ---- Reductions ---------------------------------------------------------- */
 
 /* Replaces moonlab_cuda_prob_z (cuda_statevec.h:120-128): P(qubit = 1). */

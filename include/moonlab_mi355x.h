@@ -43,6 +43,7 @@ typedef struct {
     int return_expect_z;       /* 1: fill `expect_z` (n doubles) and `norm` */
     int fusion_max_qubits;     /* 0: one kernel per gate; 1..5: fuse into k-qubit blocks */
     int skip_shots;            /* 1: ignore the job's shot count (state outputs only) */
+    int blocked;               /* 1: cache-blocked passes (moonlab_hip_apply_gates_blocked) */
 } moonlab_mi355x_exec_options_t;
 
 typedef struct {
@@ -142,6 +143,70 @@ MOONLAB_API int moonlab_mi355x_hea_tfim_energies(moonlab_mi355x_batch_t *b,
                                                  const moonlab_mi355x_hea_t *instances, int count,
                                                  int precision, double *energies,
                                                  moonlab_mi355x_batch_stats_t *stats);
+
+/* ---- Serving: continuous batching with HBM-footprint admission --------------
+ *
+ * A request loop in the spirit of the control plane's CIRCUIT / SHOTS verbs
+ * (src/control/control_plane.c:587-748), which runs one synchronous CPU
+ * simulation per request.  Here requests are queued and admitted while their
+ * state footprint (2^n x 16 or 8 bytes) fits the HBM budget; small requests
+ * (at most batch_max_qubits) are co-batched into persistent batch launches,
+ * each batch taking everything admitted since the previous one was launched
+ * (two batch engines alternate, so a batch is built while another runs);
+ * larger requests run cache-blocked on their own streams.  A request's
+ * results are the same alone and in any mix. */
+
+typedef struct moonlab_mi355x_server moonlab_mi355x_server_t;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;   /* admission budget, default 256 GiB */
+    int      batch_max_qubits;   /* co-batch requests up to this size (default 16, max 24) */
+    int      max_batch;          /* requests per batch launch (default 1024) */
+    int      batch_engines;      /* concurrent batch launches (default 2) */
+    int      large_workers;      /* concurrent large requests (default 4) */
+    int      sequential;         /* 1: plain dispatch -- one request at a time, one kernel per gate */
+} moonlab_mi355x_server_config_t;
+
+typedef struct {
+    int                       num_qubits;    /* 1..34 */
+    int                       precision;     /* MOONLAB_HIP_C128 or MOONLAB_HIP_C64 */
+    const moonlab_hip_gate_t *gates;         /* copied at submit */
+    int                       num_gates;
+    int                       num_shots;     /* draws: moonlab_qgtl_execute's xorshift64 stream */
+    uint64_t                  rng_seed;      /* 0 = clock-based seed, as the reference */
+    int                       want_expect_z; /* 1: <Z_q> for every qubit and the norm */
+} moonlab_mi355x_request_t;
+
+typedef struct {
+    int       status;          /* MOONLAB_SCHED_OK or a negative code */
+    int       num_qubits;
+    int       num_shots;
+    uint64_t *outcomes;        /* malloc'd, num_shots */
+    double   *expect_z;        /* malloc'd, num_qubits, or NULL */
+    double    norm;
+    double    submit_time;     /* seconds on the server clock (moonlab_mi355x_server_now) */
+    double    start_time;
+    double    finish_time;
+    int       path;            /* 0 = co-batched, 1 = single-state, 2 = sequential */
+} moonlab_mi355x_response_t;
+
+typedef struct {
+    uint64_t submitted, completed, batches, batched_requests, large_requests;
+    uint64_t peak_bytes_admitted;
+    int      largest_batch;
+} moonlab_mi355x_server_stats_t;
+
+MOONLAB_API moonlab_mi355x_server_t *moonlab_mi355x_server_create(const moonlab_mi355x_server_config_t *cfg);
+MOONLAB_API void moonlab_mi355x_server_free(moonlab_mi355x_server_t *s);
+MOONLAB_API double moonlab_mi355x_server_now(const moonlab_mi355x_server_t *s);
+MOONLAB_API int moonlab_mi355x_server_submit(moonlab_mi355x_server_t *s, const moonlab_mi355x_request_t *req,
+                                             uint64_t *ticket);
+/* Wait for a ticket's response (ownership passes to the caller; free with
+ * moonlab_mi355x_response_free).  MOONLAB_SCHED_BACKEND_BUSY on timeout. */
+MOONLAB_API int moonlab_mi355x_server_wait(moonlab_mi355x_server_t *s, uint64_t ticket,
+                                           moonlab_mi355x_response_t *out, double timeout_s);
+MOONLAB_API void moonlab_mi355x_response_free(moonlab_mi355x_response_t *r);
+MOONLAB_API int moonlab_mi355x_server_stats(const moonlab_mi355x_server_t *s, moonlab_mi355x_server_stats_t *st);
 
 #ifdef __cplusplus
 }

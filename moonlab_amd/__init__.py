@@ -131,7 +131,8 @@ _Backend._fields_ = [("name", ctypes.c_char_p), ("execute", BACKEND_EXECUTE_FN),
 class _ExecOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("return_amplitudes", ctypes.c_int),
                 ("return_probabilities", ctypes.c_int), ("return_expect_z", ctypes.c_int),
-                ("fusion_max_qubits", ctypes.c_int), ("skip_shots", ctypes.c_int)]
+                ("fusion_max_qubits", ctypes.c_int), ("skip_shots", ctypes.c_int),
+                ("blocked", ctypes.c_int)]
 
 
 class _ExecResults(ctypes.Structure):
@@ -169,6 +170,34 @@ class _Hea(ctypes.Structure):
 
 
 OBS_Z, OBS_ZZ, OBS_X = 1, 2, 4
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("batch_max_qubits", ctypes.c_int),
+                ("max_batch", ctypes.c_int), ("batch_engines", ctypes.c_int),
+                ("large_workers", ctypes.c_int), ("sequential", ctypes.c_int)]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("precision", ctypes.c_int),
+                ("gates", ctypes.c_void_p), ("num_gates", ctypes.c_int),
+                ("num_shots", ctypes.c_int), ("rng_seed", ctypes.c_uint64),
+                ("want_expect_z", ctypes.c_int)]
+
+
+class _Response(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("num_qubits", ctypes.c_int), ("num_shots", ctypes.c_int),
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
+                ("expect_z", ctypes.POINTER(ctypes.c_double)), ("norm", ctypes.c_double),
+                ("submit_time", ctypes.c_double), ("start_time", ctypes.c_double),
+                ("finish_time", ctypes.c_double), ("path", ctypes.c_int)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("batched_requests", ctypes.c_uint64),
+                ("large_requests", ctypes.c_uint64), ("peak_bytes_admitted", ctypes.c_uint64),
+                ("largest_batch", ctypes.c_int)]
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -231,6 +260,13 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_mi355x_batch_set_config": ([vp, ip, ip], ip),
         "moonlab_mi355x_batch_run": ([vp, ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(_BatchStats)], ip),
         "moonlab_mi355x_batch_autotune": ([vp, ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_server_create": ([ctypes.POINTER(_ServerConfig)], vp),
+        "moonlab_mi355x_server_free": ([vp], None),
+        "moonlab_mi355x_server_now": ([vp], ctypes.c_double),
+        "moonlab_mi355x_server_submit": ([vp, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_uint64)], ip),
+        "moonlab_mi355x_server_wait": ([vp, ctypes.c_uint64, ctypes.POINTER(_Response), ctypes.c_double], ip),
+        "moonlab_mi355x_response_free": ([ctypes.POINTER(_Response)], None),
+        "moonlab_mi355x_server_stats": ([vp, ctypes.POINTER(_ServerStats)], ip),
         "moonlab_mi355x_hea_tfim_energies": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
         # state-vector layer
         "moonlab_hip_device_count": ([], ip),
@@ -257,6 +293,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_apply_controlled_1q": ([vp, u64, u32, dp], ip),
         "moonlab_hip_apply_dense": ([vp, ip, ctypes.POINTER(ctypes.c_int), dp], ip),
         "moonlab_hip_apply_gates_fused": ([vp, vp, ip, ip], ip),
+        "moonlab_hip_apply_gates_blocked": ([vp, vp, ip], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
         "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
@@ -423,9 +460,10 @@ class Job:
 
     def execute_ex(self, precision: int = C128, amplitudes: bool = False,
                    probabilities: bool = False, expect_z: bool = False,
-                   fusion_max_qubits: int = 0, skip_shots: bool = False) -> ExecResults:
+                   fusion_max_qubits: int = 0, skip_shots: bool = False,
+                   blocked: bool = False) -> ExecResults:
         o = _ExecOpts(precision, int(amplitudes), int(probabilities), int(expect_z),
-                      fusion_max_qubits, int(skip_shots))
+                      fusion_max_qubits, int(skip_shots), int(blocked))
         r = _ExecResults()
         _check_sched(lib().moonlab_mi355x_execute_ex(self._h, ctypes.byref(o), ctypes.byref(r)),
                      "moonlab_mi355x_execute_ex")
@@ -579,10 +617,13 @@ class StateVector:
         _check_hip(lib().moonlab_hip_state_reset(self._h), "reset")
         return self
 
-    def apply(self, gates, fuse: int = 0) -> "StateVector":
+    def apply(self, gates, fuse: int = 0, blocked: bool = False) -> "StateVector":
         g = gate_array(gates)
         if len(g):
-            if fuse:
+            if blocked:
+                _check_hip(lib().moonlab_hip_apply_gates_blocked(self._h, g.ctypes.data, len(g)),
+                           "apply_gates_blocked")
+            elif fuse:
                 _check_hip(lib().moonlab_hip_apply_gates_fused(self._h, g.ctypes.data, len(g), fuse),
                            "apply_gates_fused")
             else:
@@ -827,3 +868,73 @@ class Batch:
             ctypes.byref(st)), "hea_tfim_energies")
         self.last_stats = self._stats(st)
         return e
+
+
+# ---------------------------------------------------------------------------
+# Serving loop
+# ---------------------------------------------------------------------------
+@dataclass
+class Response:
+    status: int
+    outcomes: Optional[np.ndarray]
+    expect_z: Optional[np.ndarray]
+    norm: float
+    submit_time: float
+    start_time: float
+    finish_time: float
+    path: int
+
+
+class Server:
+    """moonlab_mi355x_server_*: continuous batching with HBM admission."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, batch_max_qubits: int = 0, max_batch: int = 0,
+                 batch_engines: int = 0, large_workers: int = 0, sequential: bool = False) -> None:
+        cfg = _ServerConfig(hbm_budget_bytes, batch_max_qubits, max_batch, batch_engines, large_workers,
+                            int(sequential))
+        self._h = lib().moonlab_mi355x_server_create(ctypes.byref(cfg))
+        if not self._h:
+            raise EngineError("moonlab_mi355x_server_create failed (no HIP device?)")
+        self._keep = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.moonlab_mi355x_server_free(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def now(self) -> float:
+        return lib().moonlab_mi355x_server_now(self._h)
+
+    def submit(self, n: int, gates, shots: int = 1024, seed: int = 1, precision: int = C128,
+               expect_z: bool = True) -> int:
+        g = gate_array(gates)
+        req = _Request(n, precision, g.ctypes.data if len(g) else None, len(g), shots, seed, int(expect_z))
+        t = ctypes.c_uint64(0)
+        _check_sched(lib().moonlab_mi355x_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)), "submit")
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> Response:
+        r = _Response()
+        rc = lib().moonlab_mi355x_server_wait(self._h, ticket, ctypes.byref(r), timeout)
+        try:
+            if rc != 0:
+                raise SchedulerError(f"server_wait: status {rc}")
+            outc = np.ctypeslib.as_array(r.outcomes, (r.num_shots,)).copy() if r.num_shots else None
+            z = np.ctypeslib.as_array(r.expect_z, (r.num_qubits,)).copy() if r.expect_z else None
+            return Response(r.status, outc, z, r.norm, r.submit_time, r.start_time, r.finish_time, r.path)
+        finally:
+            lib().moonlab_mi355x_response_free(ctypes.byref(r))
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check_sched(lib().moonlab_mi355x_server_stats(self._h, ctypes.byref(st)), "server_stats")
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}

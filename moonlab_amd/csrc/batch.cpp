@@ -26,9 +26,13 @@
 
 #include "../../include/moonlab_mi355x.h"
 #include "batch_device.h"
+#include "batch_plan.h"
 #include "fusion.h"
 
 namespace mlx {
+
+hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int max_blocks,
+                            hipStream_t st);
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
@@ -36,14 +40,6 @@ hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int
                         hipStream_t st);
 
 namespace {
-
-// An op before tile assignment: global qubit indices.
-struct GOp {
-    int      kind;
-    int      t1, t2;        // targets (MAT1: t1; SWAP: t1, t2); DIAG: qa = t1, qb = t2 (-1 = none)
-    uint64_t ctrl;          // control qubits (global bits)
-    double   m[8];
-};
 
 bool is_identity(const cd u[4])
 {
@@ -102,6 +98,8 @@ void diag2_op(std::vector<GOp> &out, int qa, int qb, uint64_t ctrl, cd d0, cd d1
     }
     out.push_back(o);
 }
+
+}  // namespace
 
 // Steps 1 and 2: gate list -> global ops.
 bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out)
@@ -165,12 +163,16 @@ bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out)
     return true;
 }
 
+namespace {
+
 uint64_t needs_local(const GOp &o)
 {
     if (o.kind == BOP_MAT1) return 1ull << o.t1;
     if (o.kind == BOP_SWAP) return (1ull << o.t1) | (1ull << o.t2);
     return 0;
 }
+
+}  // namespace
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass> &passes,
@@ -228,6 +230,8 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass>
     }
     close();
 }
+
+namespace {
 
 double now_s()
 {
@@ -432,6 +436,39 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
 }  // namespace
 
 extern "C" {
+
+moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g,
+                                                     int ng)
+{
+    if (!s || ng < 0 || (ng > 0 && !g)) return MOONLAB_HIP_ERR_BAD_ARG;
+    if (ng == 0) return MOONLAB_HIP_OK;
+    const int n = (int)moonlab_hip_state_n_qubits(s);
+    std::vector<GOp> gops;
+    if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
+    std::vector<BPass> passes;
+    std::vector<BOp> ops;
+    const int tb = n < kBatchMaxTileBits ? n : kBatchMaxTileBits;
+    plan_passes(gops, n, tb, passes, ops);
+    if (ops.empty()) return MOONLAB_HIP_OK;
+    hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    BOp *d_ops = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d_ops, ops.size() * sizeof(BOp), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // `ops` is a local vector
+    for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
+        e = launch_tile_pass(moonlab_hip_state_device_ptr(s), moonlab_hip_state_precision(s), n, passes[i], d_ops,
+                             cus * 2, st);
+    if (d_ops) hipFreeAsync(d_ops, st);
+    if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
+    return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
+}
 
 moonlab_mi355x_batch_t *moonlab_mi355x_batch_create(void)
 {

@@ -81,7 +81,7 @@ __device__ __forceinline__ int bit_of(int32_t code, int l, uint64_t base)
     return code >= 0 ? (l >> code) & 1 : (int)((base >> (-1 - code)) & 1ull);
 }
 
-template <typename R>
+template <typename R, int STRIDE = NT>
 __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
 {
     if ((base & op.cg) != op.cg) return;          // a global control is 0 for this whole tile
@@ -90,7 +90,7 @@ __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
         const R m0 = (R)op.m[0], m1 = (R)op.m[1], m2 = (R)op.m[2], m3 = (R)op.m[3];
         const R m4 = (R)op.m[4], m5 = (R)op.m[5], m6 = (R)op.m[6], m7 = (R)op.m[7];
         const int half = T >> 1;
-        for (int p = tid; p < half; p += NT) {
+        for (int p = tid; p < half; p += STRIDE) {
             const int i0 = (int)ins0((uint64_t)p, op.lq), i1 = i0 | (1 << op.lq);
             if ((i0 & (int)op.cl) != (int)op.cl) continue;
             const cplx_t<R> a = t[i0], b = t[i1];
@@ -103,14 +103,14 @@ __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
             t[i1] = y;
         }
     } else if (op.kind == BOP_DIAG) {
-        for (int l = tid; l < T; l += NT) {
+        for (int l = tid; l < T; l += STRIDE) {
             if ((l & (int)op.cl) != (int)op.cl) continue;
             const int k = (bit_of(op.lq2, l, base) << 1) | bit_of(op.lq, l, base);
             t[l] = cmul(t[l], (R)op.m[2 * k], (R)op.m[2 * k + 1]);
         }
     } else {   // BOP_SWAP
         const int b1 = 1 << op.lq, b2 = 1 << op.lq2;
-        for (int l = tid; l < T; l += NT) {
+        for (int l = tid; l < T; l += STRIDE) {
             if ((l & b1) && !(l & b2) && (l & (int)op.cl) == (int)op.cl) {
                 const int j = (l ^ b1) | b2;
                 const cplx_t<R> x = t[l];
@@ -449,7 +449,56 @@ __global__ __launch_bounds__(NT) void k_batch(const BInst *__restrict__ insts,
 
 __global__ void k_reset_counter(int *c) { *c = 0; }
 
+// One cache-blocked pass over a large state: blocks walk the tiles of the
+// pass's tile set (grid-stride), each tile streamed HBM -> LDS, every op of
+// the pass applied in LDS, then streamed back.
+constexpr int kPassThreads = 1024;
+
+template <typename R>
+__global__ __launch_bounds__(kPassThreads) void k_tile_pass(cplx_t<R> *__restrict__ a, BPass P,
+                                                            const BOp *__restrict__ ops, uint64_t ntiles)
+{
+    __shared__ cplx_t<R> t[1 << kBatchMaxTileBits];
+    __shared__ uint64_t dlo[64], dhi[64];
+    const int tid = threadIdx.x;
+    const int T = 1 << P.nbits;
+    if (tid < 128) {
+        const int j = tid & 63, hi = tid >> 6;
+        uint64_t o = 0;
+        for (int b = 0; b < 6; b++) {
+            const int qi = hi * 6 + b;
+            if (qi < P.nbits && ((j >> b) & 1)) o |= 1ull << P.qubits[qi];
+        }
+        if (hi) dhi[j] = o;
+        else dlo[j] = o;
+    }
+    __syncthreads();
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const uint64_t base = tile_base(ti, P);
+        for (int l = tid; l < T; l += kPassThreads) t[l] = a[base | dlo[l & 63] | dhi[l >> 6]];
+        __syncthreads();
+        for (int k = 0; k < P.num_ops; k++) {
+            apply_op<R, kPassThreads>(t, T, ops[P.first_op + k], base);
+            __syncthreads();
+        }
+        for (int l = tid; l < T; l += kPassThreads) a[base | dlo[l & 63] | dhi[l >> 6]] = t[l];
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int max_blocks,
+                            hipStream_t st)
+{
+    const uint64_t ntiles = 1ull << (n - P.nbits);
+    const unsigned grid = (unsigned)(ntiles < (uint64_t)max_blocks ? ntiles : (uint64_t)max_blocks);
+    if (prec == 1)
+        k_tile_pass<float><<<grid, kPassThreads, 0, st>>>((float2 *)amps, P, d_ops, ntiles);
+    else
+        k_tile_pass<double><<<grid, kPassThreads, 0, st>>>((double2 *)amps, P, d_ops, ntiles);
+    return hipGetLastError();
+}
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,

@@ -1,0 +1,31 @@
+// batch_plan.h -- gate-list lowering and cache-blocked pass planning shared by
+// the persistent batch engine and the blocked single-state path.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/moonlab_hip_statevec.h"
+#include "batch_device.h"
+
+namespace mlx {
+
+// An op before tile assignment: global qubit indices.
+struct GOp {
+    int      kind;          // BOP_*
+    int      t1, t2;        // targets (MAT1: t1; SWAP: t1, t2); DIAG: qa = t1, qb = t2 (-1 = none)
+    uint64_t ctrl;          // control qubits (global bits)
+    double   m[8];
+};
+
+// Gate list -> ops, folding each qubit's run of single-qubit gates into one
+// FP64 2x2.  False on an invalid gate or qubit.
+bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out);
+
+// Pack ops into passes over tile sets of `tb` qubits (n <= tb: one resident
+// pass over all n qubits) and convert qubits to tile-local indices.
+void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass> &passes,
+                 std::vector<BOp> &ops);
+
+}  // namespace mlx

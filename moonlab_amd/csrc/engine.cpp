@@ -140,24 +140,31 @@ void release_state(moonlab_hip_state_t *s)
 
 int run_circuit(const RunRequest &rq, RunResult &rr)
 {
-    moonlab_hip_state_t *s = nullptr;
-    int st = acquire_state(rq.num_qubits, rq.precision, &s);
-    if (st) return st;
-
     std::vector<moonlab_hip_gate_t> g((size_t)rq.num_gates);
     for (int i = 0; i < rq.num_gates; i++) {
         const GateRec &r = rq.gates[i];
         g[(size_t)i] = moonlab_hip_gate_t{r.type, r.target, r.control, -1, r.theta, 0.0, 0.0};
     }
+    return run_circuit_gates(rq, g.data(), rq.num_gates, rr);
+}
+
+int run_circuit_gates(const RunRequest &rq, const moonlab_hip_gate_t *gl, int ng, RunResult &rr)
+{
+    moonlab_hip_state_t *s = nullptr;
+    int st = acquire_state(rq.num_qubits, rq.precision, &s);
+    if (st) return st;
+    std::vector<moonlab_hip_gate_t> g(gl, gl + ng);
     hipStream_t stream = (hipStream_t)moonlab_hip_state_stream(s);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     hipEventRecord(e0, stream);
-    if (rq.fusion_max > 0)
-        st = apply_fused(s, g.data(), rq.num_gates, rq.fusion_max);
+    if (rq.blocked)
+        st = moonlab_hip_apply_gates_blocked(s, g.data(), ng);
+    else if (rq.fusion_max > 0)
+        st = apply_fused(s, g.data(), ng, rq.fusion_max);
     else
-        st = moonlab_hip_apply_gates(s, g.data(), rq.num_gates);
+        st = moonlab_hip_apply_gates(s, g.data(), ng);
     if (!st && rq.num_draws > 0) {
         rr.outcomes.resize((size_t)rq.num_draws);
         st = moonlab_hip_sample(s, rq.draws, rq.num_draws, rr.outcomes.data());
@@ -272,6 +279,7 @@ int execute_common(const moonlab_job_t *job, const moonlab_mi355x_exec_options_t
     rq.want_probs = opts && opts->return_probabilities;
     rq.want_z = opts && opts->return_expect_z;
     rq.fusion_max = opts ? opts->fusion_max_qubits : 0;
+    rq.blocked = opts && opts->blocked;
     const int st = run_circuit(rq, rr);
     if (st) {
         fprintf(stderr, "moonlab-mi355x: %s\n", moonlab_hip_last_error());

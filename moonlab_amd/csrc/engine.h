@@ -46,6 +46,7 @@ struct RunRequest {
     bool                want_probs;
     bool                want_z;
     int                 fusion_max;     // 0 = unfused
+    bool                blocked;        // cache-blocked tile passes
 };
 
 struct RunResult {
@@ -60,5 +61,7 @@ struct RunResult {
 // Run a gate list from |0..0> on a pooled device state.  Returns a
 // moonlab_hip_status_t value; the text is in moonlab_hip_last_error().
 int run_circuit(const RunRequest &rq, RunResult &rr);
+// The same with gates of the full code table (rq.gates is ignored).
+int run_circuit_gates(const RunRequest &rq, const moonlab_hip_gate_t *gates, int num_gates, RunResult &rr);
 
 }  // namespace mlx

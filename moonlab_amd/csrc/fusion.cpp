@@ -18,6 +18,7 @@
 namespace mlx {
 
 hipError_t launch_dense(const SvDev &s, int k, const int *qubits, const double *mat);
+hipError_t launch_dense_dev(const SvDev &s, int k, const int *qubits, const void *d_mat);
 
 namespace {
 
@@ -211,18 +212,55 @@ int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int
     if (kmax < 1) kmax = 1;
     if (kmax > 5) kmax = 5;
     const std::vector<FusedBlock> blocks = plan_fusion(g, ng, kmax);
-    std::vector<double> m;
-    for (const FusedBlock &b : blocks) {
-        const int k = (int)b.qubits.size();
-        m.resize(b.mat.size() * 2);
-        for (size_t i = 0; i < b.mat.size(); i++) {
-            m[2 * i] = b.mat[i].real();
-            m[2 * i + 1] = b.mat[i].imag();
+    const SvDev d{moonlab_hip_state_device_ptr(s), (int)moonlab_hip_state_n_qubits(s), moonlab_hip_state_precision(s),
+                  (hipStream_t)moonlab_hip_state_stream(s)};
+    const bool f = d.prec == MOONLAB_HIP_C64;
+    const size_t ab = f ? 8 : 16;
+    // every matrix of 3+ qubits goes to the device in one copy
+    std::vector<size_t> off(blocks.size(), 0);
+    size_t total = 0;
+    for (size_t i = 0; i < blocks.size(); i++)
+        if (blocks[i].qubits.size() >= 3) {
+            off[i] = total;
+            total += blocks[i].mat.size() * ab;
         }
-        const int st = moonlab_hip_apply_dense(s, k, b.qubits.data(), m.data());
-        if (st) return st;
+    std::vector<unsigned char> host(total);
+    for (size_t i = 0; i < blocks.size(); i++) {
+        if (blocks[i].qubits.size() < 3) continue;
+        const std::vector<cd> &m = blocks[i].mat;
+        if (f) {
+            float *p = (float *)(host.data() + off[i]);
+            for (size_t j = 0; j < m.size(); j++) { p[2 * j] = (float)m[j].real(); p[2 * j + 1] = (float)m[j].imag(); }
+        } else {
+            double *p = (double *)(host.data() + off[i]);
+            for (size_t j = 0; j < m.size(); j++) { p[2 * j] = m[j].real(); p[2 * j + 1] = m[j].imag(); }
+        }
     }
-    return MOONLAB_HIP_OK;
+    void *dm = nullptr;
+    hipError_t e = hipSuccess;
+    if (total) {
+        e = hipMallocAsync(&dm, total, d.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dm, host.data(), total, hipMemcpyHostToDevice, d.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(d.stream);   // `host` is local
+    }
+    std::vector<double> m2;
+    for (size_t i = 0; e == hipSuccess && i < blocks.size(); i++) {
+        const FusedBlock &b = blocks[i];
+        const int k = (int)b.qubits.size();
+        if (k >= 3) {
+            e = launch_dense_dev(d, k, b.qubits.data(), (const char *)dm + off[i]);
+            continue;
+        }
+        m2.resize(b.mat.size() * 2);
+        for (size_t j = 0; j < b.mat.size(); j++) {
+            m2[2 * j] = b.mat[j].real();
+            m2[2 * j + 1] = b.mat[j].imag();
+        }
+        e = k == 1 ? launch_matrix_1q(d, b.qubits[0], m2.data()) : launch_matrix_2q(d, b.qubits[0], b.qubits[1], m2.data());
+    }
+    if (dm) hipFreeAsync(dm, d.stream);
+    if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
+    return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
 }
 
 }  // namespace mlx

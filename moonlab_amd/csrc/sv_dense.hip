@@ -62,9 +62,15 @@ hipError_t run_dense(const SvDev &s, const int *qubits, const void *d_mat)
 
 }  // namespace
 
-// d_mat: device copy of the 2^k x 2^k matrix in the state's precision.
+bool mfma_dense_supported(const SvDev &s, int k);
+hipError_t launch_dense_mfma(const SvDev &s, int k, const int *qubits, const void *d_mat);
+
+// d_mat: device copy of the 2^k x 2^k matrix in the state's precision.  The
+// matrix cores take every block whose tile (2^(k+6) amplitudes) fits the
+// state; tiny states use the VALU kernel.
 hipError_t launch_dense_dev(const SvDev &s, int k, const int *qubits, const void *d_mat)
 {
+    if (mfma_dense_supported(s, k)) return launch_dense_mfma(s, k, qubits, d_mat);
     const bool f = s.prec == MOONLAB_HIP_C64;
     switch (k) {
     case 3: return f ? run_dense<float, 3>(s, qubits, d_mat) : run_dense<double, 3>(s, qubits, d_mat);

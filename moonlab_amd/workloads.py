@@ -97,3 +97,52 @@ def algorithmic_bytes(gates: np.ndarray, n: int, amp_bytes: int = 16) -> int:
         else:
             total += 2 * D * amp_bytes
     return total
+
+
+def hea_batch(count: int, seed: int, n_range=(8, 18), layer_range=(4, 12), h_range=(0.5, 1.5)):
+    """Hardware-efficient-ansatz VQE instances: n ~ U{n_range}, layers ~
+    U{layer_range}, angles ~ U[0, 2 pi), transverse field h ~ U[h_range]."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = n_range[0] + rng.below(n_range[1] - n_range[0] + 1)
+        L = layer_range[0] + rng.below(layer_range[1] - layer_range[0] + 1)
+        ang = np.array([2.0 * math.pi * rng.unit() for _ in range(L * n * 2)]).reshape(L, n, 2)
+        h = h_range[0] + (h_range[1] - h_range[0]) * rng.unit()
+        out.append((n, L, ang, h))
+    return out
+
+
+def hea_gate_list(n: int, L: int, ang: np.ndarray) -> np.ndarray:
+    rows = []
+    for l in range(L):
+        for q in range(n):
+            rows.append((int(GateType.RY), q, -1, -1, float(ang[l, q, 0]), 0.0, 0.0))
+            rows.append((int(GateType.RZ), q, -1, -1, float(ang[l, q, 1]), 0.0, 0.0))
+        for q in range(n - 1):
+            rows.append((int(GateType.CZ), q + 1, q, -1, 0.0, 0.0, 0.0))
+    return np.array(rows, dtype=GATE_DTYPE)
+
+
+def serving_requests(count: int, seed: int, rate_hz: float, n_range=(10, 30), depth_range=(10, 50),
+                     shots: int = 1024):
+    """Poisson stream of random-layer circuits: n in n_range with P(n)
+    proportional to 2^(-(n - n_min)/4), depth ~ U{depth_range}, precision
+    complex128 / complex64 at 50/50, `shots` shots + <Z_q> each.  Returns
+    dicts with arrival time (s), n, depth, precision, shot seed and gates."""
+    rng = SplitMix64(seed)
+    ns = list(range(n_range[0], n_range[1] + 1))
+    w = np.array([2.0 ** (-(n - n_range[0]) / 4.0) for n in ns])
+    cdf = np.cumsum(w / w.sum())
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += -math.log(1.0 - rng.unit()) / rate_hz
+        n = ns[int(np.searchsorted(cdf, rng.unit(), side="right").clip(0, len(ns) - 1))]
+        depth = depth_range[0] + rng.below(depth_range[1] - depth_range[0] + 1)
+        prec = int(rng.below(2))
+        cseed = rng.next_u64()
+        sseed = rng.next_u64() | 1
+        out.append({"arrival": t, "n": n, "depth": depth, "precision": prec, "shot_seed": sseed,
+                    "shots": shots, "gates": random_layer_circuit(n, depth, cseed)})
+    return out

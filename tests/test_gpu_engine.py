@@ -362,3 +362,29 @@ def test_large_state_properties(eng):
         z, nrm = s.expect_z_all()
     assert abs(head[0] - 1.0) < 1e-12 and np.max(np.abs(head[1:])) < 1e-12
     assert np.max(np.abs(z - 1.0)) < 1e-12
+
+
+def test_cache_blocked_application_vs_oracle(eng, oracle):
+    """Cache-blocked passes (12-qubit LDS tiles) over the whole gate table."""
+    rng = np.random.default_rng(41)
+    for n in (1, 3, 9, 12, 13, 15, 18):
+        gl = random_gate_list(n, 150, rng)
+        want = oracle.run(n, gl)
+        with eng.StateVector(n) as s:
+            got = s.apply(gl, blocked=True).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL, n
+        with eng.StateVector(n, eng.C64) as s:
+            got = s.apply(gl, blocked=True).amplitudes()
+        assert abs(np.vdot(want, got)) ** 2 > 1 - 1e-6, n
+    from moonlab_amd import workloads
+    n = 20
+    g = workloads.random_layer_circuit(n, 20, 20261202)
+    want = oracle.run(n, g)
+    r = None
+    j = eng.Job(n).set_num_shots(300).set_rng_seed(77)
+    for row in g:
+        t = int(row["type"])
+        j.add_gate(t, int(row["target"]), int(row["control"]), float(row["theta"]) if t in (7, 8, 9) else None)
+    r = j.execute_ex(amplitudes=True, expect_z=True, blocked=True)
+    assert np.max(np.abs(r.amplitudes - want)) < TOL
+    assert r.outcomes.tolist() == oracle.job_shots(n, want, 300, 1, 77).tolist()
