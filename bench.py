@@ -39,9 +39,24 @@ def kernel_of(t: int) -> str:
     return "other"
 
 
-def cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | None:
+def max_amp_error(m, n: int, gates: np.ndarray, ref_amps) -> float:
+    """max |a_gpu - a_ref| over all 2^n amplitudes, the GPU state re-run on
+    the same gate prefix and read back in 2^24-amplitude chunks."""
+    worst = 0.0
+    with m.StateVector(n, m.C128) as st:
+        st.apply(gates)
+        chunk = 1 << min(n, 24)
+        for off in range(0, 1 << n, chunk):
+            a = st.read(off, chunk)
+            worst = max(worst, float(np.max(np.abs(a - ref_amps[off:off + chunk]))))
+    return worst
+
+
+def cpu_baseline(n: int, gates: np.ndarray, budget_s: float, m=None) -> dict | None:
     """The reference's C (oracle/_ref) on the host: a prefix of the same sweep,
-    gate by gate, until `budget_s` of CPU work; state init excluded."""
+    gate by gate, until `budget_s` of CPU work; state init excluded.  With the
+    engine module `m`, the GPU's state after the same prefix is compared with
+    the reference's amplitude by amplitude."""
     try:
         from oracle.oracle import RefLib, gates_array
         from moonlab_amd.workloads import algorithmic_bytes
@@ -62,6 +77,12 @@ def cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | None:
                 raise RuntimeError(f"reference gate {done} failed ({rc})")
             done += 1
         dt = time.perf_counter() - t0
+        err = None
+        if m is not None and done > 0:
+            D = 1 << n
+            ptr = ref.L.ref_state_amplitudes(s)
+            ref_amps = np.ctypeslib.as_array(ptr, shape=(2 * D,)).view(np.complex128)
+            err = max_amp_error(m, n, gates[:done], ref_amps)
     finally:
         ref.free(s)
     b = algorithmic_bytes(gates[:done], n)
@@ -70,7 +91,8 @@ def cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | None:
             "sample": f"first {done} of {len(gates)} gates of the same {n}-qubit sweep "
                       f"({b / 1e9:.1f} GB algorithmic, {dt:.1f} s), tsotchke/moonlab C "
                       f"(gates.c stride-block loops, OpenMP) built by oracle/Makefile",
-            "gates_per_s": round(done / dt, 3)}
+            "gates_per_s": round(done / dt, 3),
+            "max_abs_amp_error_gpu_vs_reference": err}
 
 
 def batch_cpu_baseline(inst, budget_s: float) -> dict | None:
@@ -191,6 +213,76 @@ def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
             "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2 ** 30, 3)}
 
 
+def run_fusion_workload(m, depth: int, cases) -> dict:
+    """Precision + fusion: depth-`depth` random SU(2) + CZ brickwork circuits,
+    fused into <= 5-qubit blocks (composed in FP64) and contracted on the
+    matrix cores (FP32 MFMA for complex64, FP64 MFMA for complex128).  The
+    FP64 comparison state is the cache-blocked complex128 path, which shares
+    neither the fusion planner nor the MFMA kernel."""
+    from moonlab_amd import workloads
+    out = {}
+    with m.StateVector(12, m.C64) as w:         # load the code objects outside the timing
+        w.apply(workloads.su2_cz_circuit(12, 2, 1), fuse=5)
+        w.synchronize()
+    for n, prec in cases:
+        g = workloads.su2_cz_circuit(n, depth, SEED + n)
+        plan = m.fusion_plan(g, 5)
+        ks = [len(b.qubits) for b in plan]
+        D = 1 << n
+        amp_b = 8 if prec == m.C64 else 16
+        dense_flops = sum(8 * (1 << k) * D for k in ks if k >= 3)
+        passes = len(plan)
+        print(f"[bench] fusion n={n} {'c64' if prec == m.C64 else 'c128'}: {len(g)} gates -> {passes} blocks",
+              file=sys.stderr, flush=True)
+        st = m.StateVector(n, prec)
+        st.timer_begin(2)
+        st.timer_mark()
+        st.apply(g, fuse=5)
+        st.timer_mark()
+        t_fused = float(st.timer_elapsed(1)[0]) * 1e-3
+        z, nrm = st.expect_z_all()
+        print(f"[bench] fusion n={n}: fused run {t_fused:.2f} s", file=sys.stderr, flush=True)
+        fid = None
+        try:
+            ref = m.StateVector(n, m.C128)
+        except m.EngineError:
+            ref = None
+        if ref is None:                           # both states do not fit: compare <Z> only
+            st.free()
+            st = None
+            ref = m.StateVector(n, m.C128)
+        ref.timer_begin(2)
+        ref.timer_mark()
+        ref.apply(g, blocked=True)
+        ref.timer_mark()
+        t_ref = float(ref.timer_elapsed(1)[0]) * 1e-3
+        z_ref, nrm_ref = ref.expect_z_all()
+        if st is not None:
+            ov = st.inner(ref)
+            fid = abs(ov) ** 2 / (nrm * nrm_ref)
+            st.free()
+        ref.free()
+        print(f"[bench] fusion n={n}: FP64 comparison run {t_ref:.2f} s", file=sys.stderr, flush=True)
+        peak = 157.3 if prec == m.C64 else 78.6
+        tflops = dense_flops / t_fused / 1e12
+        out[f"{n}q_{'c64' if prec == m.C64 else 'c128'}"] = {
+            "gates": len(g), "fused_blocks": passes,
+            "block_qubits_hist": {str(k): ks.count(k) for k in sorted(set(ks))},
+            "device_seconds": round(t_fused, 4),
+            "ms_per_block": round(t_fused / passes * 1e3, 3),
+            "state_TB_per_s": round(passes * 2 * D * amp_b / t_fused / 1e12, 3),
+            "mfma_TFLOP_per_s": round(tflops, 2),
+            "mfma_peak_TFLOP_per_s": peak,
+            "mfma_frac": round(tflops / peak, 4),
+            "fidelity_vs_fp64": fid,
+            "max_abs_dz_vs_fp64": float(np.max(np.abs(np.asarray(z) - np.asarray(z_ref)))),
+            "norm": nrm,
+            "fp64_comparison": "cache-blocked complex128 path",
+            "fp64_comparison_seconds": round(t_ref, 3),
+        }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +298,11 @@ def main() -> int:
     ap.add_argument("--serve-count", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--workload", choices=("sweep", "fusion"), default="sweep",
+                    help="sweep: the 30-qubit gate sweep (with the batch and serving legs); "
+                         "fusion: 32q complex64 and 33q complex128 SU(2)+CZ depth 60 on fused MFMA blocks")
+    ap.add_argument("--fusion-depth", type=int, default=60)
+    ap.add_argument("--fusion-cases", default="32:c64,33:c128")
     ap.add_argument("--probe", action="store_true",
                     help="short run for profiler counters: one device copy of the state "
                          "(known bytes, the calibration kernel) and the first layer of the sweep")
@@ -217,6 +314,24 @@ def main() -> int:
 
     import moonlab_amd as m
     from moonlab_amd import workloads
+
+    if args.workload == "fusion":
+        cases = [(int(c.split(":")[0]), m.C64 if c.split(":")[1] == "c64" else m.C128)
+                 for c in args.fusion_cases.split(",")]
+        t0 = time.perf_counter()
+        res = run_fusion_workload(m, args.fusion_depth, cases)
+        wall = time.perf_counter() - t0
+        first = next(iter(res.values()))
+        print(json.dumps({
+            "metric": "fidelity and <Z> error vs FP64, MFMA TFLOP/s on fused SU(2)+CZ circuits",
+            "value": first["mfma_TFLOP_per_s"], "unit": "TFLOP/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(wall * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+            "config": {"workload": f"random SU(2) per qubit + CZ brickwork, depth {args.fusion_depth}, "
+                                   "fused into <= 5-qubit blocks (FP64 composition) on MFMA",
+                       "cases": args.fusion_cases},
+            "cases": res}), flush=True)
+        return 0
 
     n = args.qubits
     gates = workloads.random_layer_circuit(n, args.depth, SEED, h_layer=True)
@@ -288,7 +403,7 @@ def main() -> int:
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
 
     value = bytes_step * K / wall / 1e12
-    cpu = None if args.no_cpu else cpu_baseline(n, gates, args.cpu_seconds)
+    cpu = None if args.no_cpu else cpu_baseline(n, gates, args.cpu_seconds, m)
     batch = None
     if not args.no_batch:
         batch = run_batch_workload(m, args.batch_count, args.batch_steps, 1, 15.0, not args.no_cpu)

@@ -157,6 +157,16 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_fused(moonlab_hip_state
                                                                const moonlab_hip_gate_t *gates,
                                                                int num_gates, int kmax);
 
+/* Addition: the fusion plan moonlab_hip_apply_gates_fused would execute,
+ * computed on the host (no device needed).  Block i acts on the qubits set in
+ * masks[i] (its local basis bit j is the j-th lowest of them) and holds
+ * counts[i] gates; when mats is not NULL its FP64 unitary (2^k x 2^k,
+ * row-major (re, im) doubles) is written at mats + 2048 i.  Qubits must lie
+ * in 0..63.  Returns the block count (only the first max_blocks are written)
+ * or a negative moonlab_hip_status_t value. */
+MOONLAB_API int moonlab_hip_fusion_plan(const moonlab_hip_gate_t *gates, int num_gates, int kmax,
+                                        uint64_t *masks, int *counts, double *mats, int max_blocks);
+
 /* Addition: apply a gate list cache-blocked: single-qubit runs fold into one
  * FP64 2x2 per qubit, then ops are packed into passes whose non-diagonal
  * targets fit a 12-qubit tile set; each pass streams the state through LDS
@@ -189,7 +199,8 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_correlation_zz(const moonlab_hip_st
 /* Addition: |a_i|^2 for all i into a host buffer of 2^n doubles. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_probabilities(const moonlab_hip_state_t *state,
                                                            double *out);
-/* Addition: <a|b> (same n and precision). */
+/* Addition: <a|b> (same n; either precision, products and sums in FP64, so a
+ * complex64 run can be compared with a complex128 one without a copy). */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_inner_product(const moonlab_hip_state_t *a,
                                                            const moonlab_hip_state_t *b,
                                                            double *out_re, double *out_im);

@@ -293,6 +293,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_apply_controlled_1q": ([vp, u64, u32, dp], ip),
         "moonlab_hip_apply_dense": ([vp, ip, ctypes.POINTER(ctypes.c_int), dp], ip),
         "moonlab_hip_apply_gates_fused": ([vp, vp, ip, ip], ip),
+        "moonlab_hip_fusion_plan": ([vp, ip, ip, u64p, ctypes.POINTER(ctypes.c_int), dp, ip], ip),
         "moonlab_hip_apply_gates_blocked": ([vp, vp, ip], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
@@ -762,6 +763,37 @@ class StateVector:
         if n < 0:
             raise EngineError(f"timer_elapsed failed: {last_error()}")
         return out[:n]
+
+
+@dataclass
+class FusedBlock:
+    qubits: List[int]
+    num_gates: int
+    matrix: np.ndarray     # 2^k x 2^k complex128, local bit j = qubits[j]
+
+
+def fusion_plan(gates, kmax: int = 5) -> List[FusedBlock]:
+    """The blocks StateVector.apply(gates, fuse=kmax) executes, in order,
+    with their FP64 unitaries (host only, no device needed)."""
+    g = gate_array(gates)
+    L = lib()
+    nb = L.moonlab_hip_fusion_plan(g.ctypes.data, len(g), kmax, None, None, None, 0)
+    if nb < 0:
+        raise EngineError(f"fusion_plan failed ({nb}): {last_error()}")
+    masks = np.zeros(max(nb, 1), dtype=np.uint64)
+    counts = np.zeros(max(nb, 1), dtype=np.int32)
+    mats = np.zeros((max(nb, 1), 2048), dtype=np.float64)
+    L.moonlab_hip_fusion_plan(g.ctypes.data, len(g), kmax,
+                              masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                              counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              mats.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nb)
+    out = []
+    for i in range(nb):
+        qs = [q for q in range(64) if (int(masks[i]) >> q) & 1]
+        d = 1 << len(qs)
+        m = mats[i, :2 * d * d].view(np.complex128).reshape(d, d).copy()
+        out.append(FusedBlock(qs, int(counts[i]), m))
+    return out
 
 
 def set_streaming_threshold(nbytes: int) -> None:

@@ -1,10 +1,23 @@
-"""Build the MI355X engine library (libmoonlab_mi355x.so) in-tree with hipcc.
+"""Build the MI355X engine libraries in-tree with hipcc.
 
     python -m moonlab_amd.build            # compile for gfx950
 
 Every HIP translation unit is compiled for gfx950 only, with packed-FP32
 instruction selection disabled.  Objects go to moonlab_amd/_build/ and the
-shared library to moonlab_amd/lib/.
+shared libraries to moonlab_amd/lib/:
+
+  libmoonlab_mi355x.so         the engine plus its own job / circuit API
+                               (a drop-in for the reference library's
+                               scheduler and QGTL entry points);
+  libmoonlab_mi355x_plugin.so  the engine alone, for a process that already
+                               runs the reference's scheduler: its job-API
+                               references (moonlab_job_num_*, moonlab_job_to_json,
+                               moonlab_register_backend) resolve to the host
+                               library, so moonlab_mi355x_register_backend()
+                               registers "mi355x" in the host's own registry.
+
+Both link with -Bsymbolic, so calls inside a library bind to its own
+definitions whatever else the process has loaded.
 """
 from __future__ import annotations
 
@@ -20,6 +33,9 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libmoonlab_mi355x.so")
+PLUGIN = os.path.join(LIBDIR, "libmoonlab_mi355x_plugin.so")
+# translation units that implement the job / circuit API (left out of the plugin)
+HOST_API = ("scheduler.cpp", "qgtl_circuit.cpp")
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -55,14 +71,16 @@ def build(verbose: bool = True) -> str:
     workers = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    plugin_objs = [o for o in objs if not os.path.basename(o)[:-2] in HOST_API]
+    for out, parts in ((LIB, objs), (PLUGIN, plugin_objs)):
+        tmp = out + ".tmp"
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", tmp] + parts + ["-lpthread"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link of {os.path.basename(out)} failed:\n{r.stderr}")
+        os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources and {os.path.basename(PLUGIN)} from {len(plugin_objs)}")
     return LIB
 
 

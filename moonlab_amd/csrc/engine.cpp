@@ -315,7 +315,13 @@ int moonlab_mi355x_backend_execute(const moonlab_job_t *job, moonlab_job_results
     out->num_workers_used = workers;
     out->outcomes = (uint64_t *)malloc((size_t)shots * sizeof(uint64_t));
     out->worker_seconds = (double *)calloc((size_t)workers, sizeof(double));
-    if (!out->outcomes || !out->worker_seconds) return MOONLAB_SCHED_OOM;
+    if (!out->outcomes || !out->worker_seconds) {
+        free(out->outcomes);
+        free(out->worker_seconds);
+        out->outcomes = nullptr;
+        out->worker_seconds = nullptr;
+        return MOONLAB_SCHED_OOM;
+    }
     memcpy(out->outcomes, rr.outcomes.data(), (size_t)shots * sizeof(uint64_t));
     // One device run serves every worker slice; each slot carries the
     // slice's share of the wall time.

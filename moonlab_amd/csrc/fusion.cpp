@@ -153,48 +153,112 @@ void gate_dense(const moonlab_hip_gate_t &g, cd *m)
     }
 }
 
+namespace {
+
+// Compose a block's unitary in FP64: the identity, then each member gate
+// left-multiplied in circuit order.
+void compose_block(FusedBlock &b, const std::vector<const moonlab_hip_gate_t *> &members)
+{
+    const int k = (int)b.qubits.size(), dim = 1 << k;
+    b.mat.assign((size_t)dim * dim, cd(0.0));
+    for (int i = 0; i < dim; i++) b.mat[(size_t)i * dim + i] = 1.0;
+    std::vector<cd> col((size_t)dim);
+    for (const moonlab_hip_gate_t *gp : members) {
+        if (gp->type == MOONLAB_HIP_GATE_I) continue;
+        const GateAction a = action_of(*gp);
+        for (int c = 0; c < dim; c++) {
+            for (int r = 0; r < dim; r++) col[(size_t)r] = b.mat[(size_t)r * dim + c];
+            apply_local(a, b.qubits.data(), k, col.data());
+            for (int r = 0; r < dim; r++) b.mat[(size_t)r * dim + c] = col[(size_t)r];
+        }
+    }
+    b.num_gates = (int)members.size();
+    b.only_gate = members.size() == 1 ? members.front() : nullptr;
+}
+
+}  // namespace
+
+// Blocks on disjoint qubits commute, so several blocks stay open at once, one
+// per disjoint qubit set: a gate joins (and merges) the open blocks it
+// touches when their union still fits kmax; otherwise the largest of those
+// blocks are emitted until it fits.  A block is emitted only once nothing
+// later in the circuit can join it, and every gate that shares a qubit with
+// it and comes later lands in a block emitted after it, so the emitted order
+// equals the circuit up to commuting disjoint blocks.
 std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kmax)
 {
+    struct Open {
+        uint64_t                                mask = 0;
+        std::vector<const moonlab_hip_gate_t *> members;
+        bool                                    live = false;
+    };
+    std::vector<Open> open;
+    int owner[64];
+    std::fill(owner, owner + 64, -1);
     std::vector<FusedBlock> blocks;
-    std::vector<int> cur;                       // qubits of the open block
-    std::vector<const moonlab_hip_gate_t *> members;
-    auto flush = [&]() {
-        if (members.empty()) return;
+    auto emit = [&](int id) {
+        Open &o = open[(size_t)id];
         FusedBlock b;
-        b.qubits = cur;
-        const int k = (int)cur.size(), dim = 1 << k;
-        b.mat.assign((size_t)dim * dim, cd(0.0));
-        for (int i = 0; i < dim; i++) b.mat[(size_t)i * dim + i] = 1.0;
-        std::vector<cd> col((size_t)dim);
-        for (const moonlab_hip_gate_t *gp : members) {
-            if (gp->type == MOONLAB_HIP_GATE_I) continue;
-            const GateAction a = action_of(*gp);
-            for (int c = 0; c < dim; c++) {
-                for (int r = 0; r < dim; r++) col[(size_t)r] = b.mat[(size_t)r * dim + c];
-                apply_local(a, cur.data(), k, col.data());
-                for (int r = 0; r < dim; r++) b.mat[(size_t)r * dim + c] = col[(size_t)r];
+        for (int q = 0; q < 64; q++)
+            if (o.mask & (1ull << q)) {
+                b.qubits.push_back(q);
+                owner[q] = -1;
             }
-        }
-        b.num_gates = (int)members.size();
+        compose_block(b, o.members);
         blocks.push_back(std::move(b));
-        cur.clear();
-        members.clear();
+        o.live = false;
+        o.members.clear();
+        o.mask = 0;
     };
     for (int i = 0; i < ng; i++) {
         int q[3];
         const int nq = gate_qubits(g[i], q);
-        std::vector<int> uni = cur;
-        for (int j = 0; j < nq; j++)
-            if (std::find(uni.begin(), uni.end(), q[j]) == uni.end()) uni.push_back(q[j]);
-        std::sort(uni.begin(), uni.end());
-        if ((int)uni.size() > kmax && !members.empty()) {
-            flush();
-            uni.assign(q, q + nq);
+        uint64_t sm = 0;
+        for (int j = 0; j < nq; j++) sm |= 1ull << q[j];
+        std::vector<int> touched;
+        for (int j = 0; j < nq; j++) {
+            const int o = owner[q[j]];
+            if (o >= 0 && std::find(touched.begin(), touched.end(), o) == touched.end()) touched.push_back(o);
         }
-        cur = uni;
-        members.push_back(&g[i]);
+        auto union_mask = [&]() {
+            uint64_t u = sm;
+            for (int o : touched) u |= open[(size_t)o].mask;
+            return u;
+        };
+        uint64_t u = union_mask();
+        if (__builtin_popcountll(u) > kmax) {
+            std::sort(touched.begin(), touched.end(), [&](int x, int y) {
+                const int px = __builtin_popcountll(open[(size_t)x].mask), py = __builtin_popcountll(open[(size_t)y].mask);
+                return px != py ? px > py : x < y;
+            });
+            // a gate wider than kmax (Toffoli at kmax 2) forms a block of its own
+            while (__builtin_popcountll(u) > kmax && !touched.empty()) {
+                emit(touched.front());
+                touched.erase(touched.begin());
+                u = union_mask();
+            }
+        }
+        // merge what is left into a new open block, members in circuit order
+        Open nb;
+        nb.live = true;
+        nb.mask = u;
+        std::sort(touched.begin(), touched.end());   // creation order
+        for (int o : touched) {
+            Open &src = open[(size_t)o];
+            nb.members.insert(nb.members.end(), src.members.begin(), src.members.end());
+            src.live = false;
+            src.members.clear();
+            src.mask = 0;
+        }
+        std::sort(nb.members.begin(), nb.members.end());   // pointers into g: circuit order
+        nb.members.push_back(&g[i]);
+        open.push_back(std::move(nb));
+        const int id = (int)open.size() - 1;
+        for (int qq = 0; qq < 64; qq++)
+            if (u & (1ull << qq)) owner[qq] = id;
     }
-    flush();
+    for (size_t id = 0; id < open.size(); id++)
+        if (open[id].live) emit((int)id);
     return blocks;
 }
 
@@ -220,13 +284,13 @@ int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int
     std::vector<size_t> off(blocks.size(), 0);
     size_t total = 0;
     for (size_t i = 0; i < blocks.size(); i++)
-        if (blocks[i].qubits.size() >= 3) {
+        if (blocks[i].qubits.size() >= 3 && !blocks[i].only_gate) {
             off[i] = total;
             total += blocks[i].mat.size() * ab;
         }
     std::vector<unsigned char> host(total);
     for (size_t i = 0; i < blocks.size(); i++) {
-        if (blocks[i].qubits.size() < 3) continue;
+        if (blocks[i].qubits.size() < 3 || blocks[i].only_gate) continue;
         const std::vector<cd> &m = blocks[i].mat;
         if (f) {
             float *p = (float *)(host.data() + off[i]);
@@ -247,6 +311,10 @@ int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int
     for (size_t i = 0; e == hipSuccess && i < blocks.size(); i++) {
         const FusedBlock &b = blocks[i];
         const int k = (int)b.qubits.size();
+        if (b.only_gate) {   // a lone gate keeps its own kernel (a CZ touches a quarter of the state)
+            e = launch_gate(d, *b.only_gate);
+            continue;
+        }
         if (k >= 3) {
             e = launch_dense_dev(d, k, b.qubits.data(), (const char *)dm + off[i]);
             continue;
@@ -287,6 +355,33 @@ moonlab_hip_status_t moonlab_hip_apply_dense(moonlab_hip_state_t *s, int k, cons
     }
     if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
     return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
+}
+
+int moonlab_hip_fusion_plan(const moonlab_hip_gate_t *g, int ng, int kmax, uint64_t *masks, int *counts, double *mats,
+                            int max_blocks)
+{
+    if ((ng > 0 && !g) || ng < 0 || kmax < 1 || kmax > 5 || max_blocks < 0) return MOONLAB_HIP_ERR_BAD_ARG;
+    for (int i = 0; i < ng; i++) {
+        if (g[i].type < 0 || g[i].type > MOONLAB_HIP_GATE_FREDKIN) return MOONLAB_HIP_ERR_BAD_ARG;
+        int q[3];
+        const int nq = mlx::gate_qubits(g[i], q);
+        for (int j = 0; j < nq; j++)
+            if (q[j] < 0 || q[j] > 63 || (j > 0 && q[j] == q[j - 1])) return MOONLAB_HIP_ERR_INVALID_QUBIT;
+    }
+    const std::vector<mlx::FusedBlock> blocks = mlx::plan_fusion(g, ng, kmax);
+    for (int i = 0; i < (int)blocks.size() && i < max_blocks; i++) {
+        const mlx::FusedBlock &b = blocks[(size_t)i];
+        uint64_t m = 0;
+        for (int q : b.qubits) m |= 1ull << q;
+        if (masks) masks[i] = m;
+        if (counts) counts[i] = b.num_gates;
+        if (mats)
+            for (size_t j = 0; j < b.mat.size(); j++) {
+                mats[2048 * (size_t)i + 2 * j] = b.mat[j].real();
+                mats[2048 * (size_t)i + 2 * j + 1] = b.mat[j].imag();
+            }
+    }
+    return (int)blocks.size();
 }
 
 moonlab_hip_status_t moonlab_hip_apply_gates_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng,

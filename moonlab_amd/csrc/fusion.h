@@ -18,6 +18,7 @@ struct FusedBlock {
     std::vector<int> qubits;
     std::vector<cd>  mat;
     int              num_gates = 0;
+    const moonlab_hip_gate_t *only_gate = nullptr;   // the member when there is exactly one
 };
 
 // Qubits a gate acts on (1..3); returns the count.
@@ -30,8 +31,9 @@ void gate_dense(const moonlab_hip_gate_t &g, cd *m);
 // 2x2 matrix (row-major) of a single-qubit gate; false for multi-qubit codes.
 bool gate_matrix_1q(const moonlab_hip_gate_t &g, cd u[4]);
 
-// Greedy in-order fusion into blocks of at most kmax qubits, each composed
-// in FP64 (left-multiplication in gate order).
+// Fusion into blocks of at most kmax qubits (a wider gate stands alone),
+// several open at once over disjoint qubit sets; each block is composed in
+// FP64 (left-multiplication in gate order).
 std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kmax);
 
 // Apply a gate list through fusion (returns a moonlab_hip_status_t value).

@@ -16,8 +16,11 @@
 //   * each wave contracts 16 groups: (2^(k+1)/16) row tiles x (2^(k+1)/4)
 //     k-steps of 16x16x4 MFMAs; it reads and writes only its own groups, so
 //     no barrier separates the contraction from the write-back.
-// Operand layout of 16x16x4 (lane l): A[l % 16][l / 16], B[l / 16][l % 16],
-// D[4 (l / 16) + v][l % 16], v = 0..3.
+// Operand layout of 16x16x4 (lane l): A[l % 16][l / 16], B[l / 16][l % 16];
+// accumulator v = 0..3 holds D[4 (l / 16) + v][l % 16] for FP32 but
+// D[4 v + l / 16][l % 16] for FP64.  The FP64 path loads A with its rows
+// permuted (A row i = real row 4 (i % 4) + i / 4), which puts both results in
+// the FP32 order, so one write-back serves both.
 #include "sv_device.h"
 #include "sv_internal.h"
 
@@ -37,6 +40,8 @@ template <> struct Acc<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // real-matrix row that A row i (= lane % 16) carries
+    static __device__ __forceinline__ int a_row(int i) { return i; }
 };
 template <> struct Acc<double> {
     using type = mfma_d4;
@@ -44,6 +49,7 @@ template <> struct Acc<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int a_row(int i) { return 4 * (i & 3) + (i >> 2); }
 };
 
 // Tile geometry, computed on the host.
@@ -59,6 +65,23 @@ constexpr int kThreads = 256;
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 5); }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every outstanding global load and store of the wave, which would
+// drain the prefetch of the next tile and the write-back of the last one.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Tile t's amplitude l lives at tile_base(t) | dlo[l % 64] | dhi[l / 64].
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const DenseGeom &G)
+{
+    for (int b = 0; b < G.tb; b++) t = ins0(t, G.tq[b]);
+    return t;
+}
+
 template <typename R, int K, bool NT>
 __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__ a, uint64_t ntiles,
                                                          DenseGeom G, const cplx_t<R> *__restrict__ mat)
@@ -67,6 +90,7 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
     constexpr int NRT = (2 * D) / 16;   // row tiles
     constexpr int NS = (2 * D) / 4;     // k-steps
     constexpr int T = kGroups * D;      // amplitudes per tile
+    constexpr int PER = T / kThreads;   // amplitudes each thread moves per tile
     using AccT = Acc<R>;
     __shared__ cplx_t<R> lds[T + T / 32];
     __shared__ uint64_t dlo[64], dhi[64];
@@ -88,13 +112,13 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
         offg[tid - 160] = G.offg[tid - 160];
     }
 
-    // A operands: rows 16 rt + l % 16, columns 4 s + l / 16 of the real matrix
+    // A operands: rows 16 rt + a_row(l % 16), columns 4 s + l / 16 of the real matrix
     R A[NRT][NS];
 #pragma unroll
     for (int rt = 0; rt < NRT; rt++)
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            const int row = 16 * rt + (lane & 15), col = 4 * s + (lane >> 4);
+            const int row = 16 * rt + AccT::a_row(lane & 15), col = 4 * s + (lane >> 4);
             const cplx_t<R> m = mat[(row >> 1) * D + (col >> 1)];
             const int ra = row & 1, cb = col & 1;
             A[rt][s] = ra == cb ? m.x : (ra == 0 ? -m.y : m.y);
@@ -103,18 +127,35 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
 
     const int g = wave * 16 + (lane & 15);
     const int go = offg[g];
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        uint64_t base = t;
-        for (int b = 0; b < G.tb; b++) base = ins0(base, G.tq[b]);
-        for (int l = tid; l < T; l += kThreads) lds[padded(l)] = ld<NT>(&a[base | dlo[l & 63] | dhi[l >> 6]]);
-        __syncthreads();
+    // thread tid moves tile amplitudes l = tid + kThreads i: l % 64 = tid % 64
+    const uint64_t lo = dlo[tid & 63];
+    const R *ldsr = (const R *)lds;
+
+    // software pipeline: tile t + gridDim.x is in flight from global memory
+    // while tile t is contracted and written back
+    cplx_t<R> pre[PER];
+    uint64_t t = blockIdx.x;
+    if (t < ntiles) {
+        const uint64_t base = tile_base(t, G) | lo;
+#pragma unroll
+        for (int i = 0; i < PER; i++) pre[i] = ld<NT>(&a[base | dhi[(tid >> 6) + 4 * i]]);
+    }
+    for (; t < ntiles; t += gridDim.x) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) lds[padded(tid + kThreads * i)] = pre[i];
+        lds_barrier();
+        const uint64_t nt = t + gridDim.x;
+        if (nt < ntiles) {
+            const uint64_t nbase = tile_base(nt, G) | lo;
+#pragma unroll
+            for (int i = 0; i < PER; i++) pre[i] = ld<NT>(&a[nbase | dhi[(tid >> 6) + 4 * i]]);
+        }
 
         R B[NS];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const int kk = 4 * s + (lane >> 4);
-            const cplx_t<R> v = lds[padded(offq[kk >> 1] | go)];
-            B[s] = (kk & 1) ? v.y : v.x;
+            B[s] = ldsr[2 * padded(offq[kk >> 1] | go) + (kk & 1)];
         }
         typename AccT::type acc[NRT];
 #pragma unroll
@@ -123,15 +164,18 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
 #pragma unroll
             for (int s = 0; s < NS; s++) acc[rt] = AccT::mma(A[rt][s], B[s], acc[rt]);
         }
+        // a wave reads and writes only its own 16 groups: no barrier in between
 #pragma unroll
         for (int rt = 0; rt < NRT; rt++) {
             const int r0 = 8 * rt + 2 * (lane >> 4);
             lds[padded(offq[r0] | go)] = cplx_t<R>{acc[rt][0], acc[rt][1]};
             lds[padded(offq[r0 + 1] | go)] = cplx_t<R>{acc[rt][2], acc[rt][3]};
         }
-        __syncthreads();
-        for (int l = tid; l < T; l += kThreads) st<NT>(&a[base | dlo[l & 63] | dhi[l >> 6]], lds[padded(l)]);
-        __syncthreads();
+        lds_barrier();
+        const uint64_t base = tile_base(t, G) | lo;
+#pragma unroll
+        for (int i = 0; i < PER; i++) st<NT>(&a[base | dhi[(tid >> 6) + 4 * i]], lds[padded(tid + kThreads * i)]);
+        lds_barrier();   // every wave has read its write-back values before the next tile lands
     }
 }
 
@@ -174,20 +218,6 @@ bool make_geom(int n, int k, const int *qubits, DenseGeom &G)
     return true;
 }
 
-template <typename R, int K>
-hipError_t run_mfma(const SvDev &s, const DenseGeom &G, const void *d_mat, int max_blocks, bool nt)
-{
-    const uint64_t ntiles = (1ull << s.n) >> (K + 6);
-    const unsigned grid = (unsigned)(ntiles < (uint64_t)max_blocks ? ntiles : (uint64_t)max_blocks);
-    if (nt)
-        k_dense_mfma<R, K, true><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
-                                                                  (const cplx_t<R> *)d_mat);
-    else
-        k_dense_mfma<R, K, false><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
-                                                                   (const cplx_t<R> *)d_mat);
-    return hipGetLastError();
-}
-
 int device_cus()
 {
     static int cus = 0;
@@ -198,6 +228,37 @@ int device_cus()
         if (cus <= 0) cus = 256;
     }
     return cus;
+}
+
+// Persistent grid: as many workgroups as the device holds at once, so no
+// workgroup waits for a slot while others still have tiles to take.
+template <typename R, int K, bool NT>
+int resident_blocks()
+{
+    static int cached = 0;   // one per kernel instantiation
+    if (!cached) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_mfma<R, K, NT>, kThreads, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        cached = per_cu * device_cus();
+    }
+    return cached;
+}
+
+template <typename R, int K>
+hipError_t run_mfma(const SvDev &s, const DenseGeom &G, const void *d_mat, bool nt)
+{
+    const uint64_t ntiles = (1ull << s.n) >> (K + 6);
+    const int mb = nt ? resident_blocks<R, K, true>() : resident_blocks<R, K, false>();
+    const unsigned grid = (unsigned)(ntiles < (uint64_t)mb ? ntiles : (uint64_t)mb);
+    if (nt)
+        k_dense_mfma<R, K, true><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
+                                                                  (const cplx_t<R> *)d_mat);
+    else
+        k_dense_mfma<R, K, false><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
+                                                                   (const cplx_t<R> *)d_mat);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -213,11 +274,10 @@ hipError_t launch_dense_mfma(const SvDev &s, int k, const int *qubits, const voi
     const bool f = s.prec == MOONLAB_HIP_C64;
     const uint64_t bytes = (1ull << s.n) * (f ? 8ull : 16ull);
     const bool nt = bytes >= gate_tuning().nt_min_bytes;
-    const int mb = device_cus() * 8;
     switch (k) {
-    case 3: return f ? run_mfma<float, 3>(s, G, d_mat, mb, nt) : run_mfma<double, 3>(s, G, d_mat, mb, nt);
-    case 4: return f ? run_mfma<float, 4>(s, G, d_mat, mb, nt) : run_mfma<double, 4>(s, G, d_mat, mb, nt);
-    case 5: return f ? run_mfma<float, 5>(s, G, d_mat, mb, nt) : run_mfma<double, 5>(s, G, d_mat, mb, nt);
+    case 3: return f ? run_mfma<float, 3>(s, G, d_mat, nt) : run_mfma<double, 3>(s, G, d_mat, nt);
+    case 4: return f ? run_mfma<float, 4>(s, G, d_mat, nt) : run_mfma<double, 4>(s, G, d_mat, nt);
+    case 5: return f ? run_mfma<float, 5>(s, G, d_mat, nt) : run_mfma<double, 5>(s, G, d_mat, nt);
     default: return hipErrorInvalidValue;
     }
 }

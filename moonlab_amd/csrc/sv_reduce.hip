@@ -314,9 +314,9 @@ __global__ __launch_bounds__(256) void k_zmask(const cplx_t<R> *__restrict__ a, 
     block_store<1>(v, partial);
 }
 
-template <typename R>
+template <typename R, typename R2>
 __global__ __launch_bounds__(256) void k_inner(const cplx_t<R> *__restrict__ a,
-                                               const cplx_t<R> *__restrict__ b, uint64_t dim,
+                                               const cplx_t<R2> *__restrict__ b, uint64_t dim,
                                                double *__restrict__ partial)
 {
     double v[2] = {0.0, 0.0};
@@ -435,11 +435,19 @@ hipError_t launch_inner(const SvDev &a, const SvDev &b, double *d_partial, int b
 {
     const uint64_t dim = 1ull << a.n;
     const int g = small_grid(dim, blocks);
-    if (a.prec == MOONLAB_HIP_C64)
-        k_inner<float><<<g, 256, 0, a.stream>>>((const float2 *)a.amps, (const float2 *)b.amps, dim, d_partial);
+    // products and sums in FP64 whatever the storage precision of either state
+    const bool fa = a.prec == MOONLAB_HIP_C64, fb = b.prec == MOONLAB_HIP_C64;
+    if (fa && fb)
+        k_inner<float, float><<<g, 256, 0, a.stream>>>((const float2 *)a.amps, (const float2 *)b.amps, dim, d_partial);
+    else if (fa)
+        k_inner<float, double><<<g, 256, 0, a.stream>>>((const float2 *)a.amps, (const double2 *)b.amps, dim,
+                                                        d_partial);
+    else if (fb)
+        k_inner<double, float><<<g, 256, 0, a.stream>>>((const double2 *)a.amps, (const float2 *)b.amps, dim,
+                                                        d_partial);
     else
-        k_inner<double><<<g, 256, 0, a.stream>>>((const double2 *)a.amps, (const double2 *)b.amps, dim,
-                                                 d_partial);
+        k_inner<double, double><<<g, 256, 0, a.stream>>>((const double2 *)a.amps, (const double2 *)b.amps, dim,
+                                                         d_partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_reduce_final<<<1, 64, 0, a.stream>>>(d_partial, g, 2, d_out);

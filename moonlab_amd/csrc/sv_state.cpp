@@ -416,7 +416,7 @@ moonlab_hip_status_t moonlab_hip_inner_product(const moonlab_hip_state_t *a, con
                                                double *re, double *im)
 {
     if (!a || !b) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");
-    if (a->n != b->n || a->prec != b->prec) return fail(MOONLAB_HIP_ERR_BAD_ARG, "shape mismatch");
+    if (a->n != b->n) return fail(MOONLAB_HIP_ERR_BAD_ARG, "shape mismatch");
     moonlab_hip_state_t *s = const_cast<moonlab_hip_state_t *>(a);
     moonlab_hip_status_t st = ensure_ws(s);
     if (st) return st;

@@ -93,6 +93,22 @@ def test_state_copy_and_precision_conversion(eng, oracle):
         assert _bytes_equal(t.amplitudes(), a0.astype(np.complex64).astype(np.complex128))
 
 
+def test_mixed_precision_inner_product(eng):
+    """<a|b> between a complex64 and a complex128 state: FP64 products of the
+    stored values, within 1e-12 of numpy's."""
+    rng = np.random.default_rng(22)
+    n = 14
+    a0, b0 = random_state(n, rng), random_state(n, rng)
+    with eng.StateVector(n, eng.C64) as a, eng.StateVector(n) as b:
+        a.load(a0)
+        b.load(b0)
+        a32 = a.amplitudes()
+        assert abs(a.inner(b) - np.vdot(a32, b0)) < 1e-12
+        assert abs(b.inner(a) - np.vdot(b0, a32)) < 1e-12
+        b.load(a0)
+        assert abs(abs(a.inner(b)) ** 2 - 1.0) < 1e-6
+
+
 def test_matrix_entry_points_vs_oracle(eng, oracle):
     import scipy.linalg
     rng = np.random.default_rng(3)
@@ -309,7 +325,7 @@ def test_fused_application_vs_unfused(eng, oracle):
     # a general dense block at scattered qubits
     import scipy.linalg
     n = 11
-    for qs in ([0, 1, 2, 3, 4], [1, 4, 6, 9, 10], [0, 5, 10], [2, 3, 7, 8]):
+    for qs in ([0, 1, 2, 3, 4], [1, 4, 6, 9, 10], [0, 5, 10], [2, 3, 7, 8], [6, 7, 8, 9, 10], [10, 8, 6][::-1]):
         k = len(qs)
         h = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
         u = scipy.linalg.expm(1j * (h + h.conj().T))
@@ -322,6 +338,22 @@ def test_fused_application_vs_unfused(eng, oracle):
         with eng.StateVector(n) as s:
             got = s.load(a0).apply_dense(qs, u).amplitudes()
         assert np.max(np.abs(got - want)) < TOL, qs
+        with eng.StateVector(n, eng.C64) as s:   # FP32 matrix cores
+            got = s.load(a0).apply_dense(qs, u).amplitudes()
+        assert np.max(np.abs(got - want)) < 2e-6, qs
+    # VALU path: states too small for a 2^(k+6) tile
+    n = 8
+    h = rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32))
+    u = scipy.linalg.expm(1j * (h + h.conj().T))
+    a0 = random_state(n, rng)
+    t = a0.reshape([2] * n)
+    qs = [0, 2, 3, 5, 7]
+    axes = [n - 1 - q for q in qs[::-1]]
+    moved = np.moveaxis(t, axes, list(range(5))).reshape(32, -1)
+    want = np.moveaxis((u @ moved).reshape([2] * n), list(range(5)), axes).reshape(-1)
+    with eng.StateVector(n) as s:
+        got = s.load(a0).apply_dense(qs, u).amplitudes()
+    assert np.max(np.abs(got - want)) < TOL
 
 
 def test_hea_tfim_energy_against_reference(eng):
@@ -388,3 +420,48 @@ def test_cache_blocked_application_vs_oracle(eng, oracle):
     r = j.execute_ex(amplitudes=True, expect_z=True, blocked=True)
     assert np.max(np.abs(r.amplitudes - want)) < TOL
     assert r.outcomes.tolist() == oracle.job_shots(n, want, 300, 1, 77).tolist()
+
+
+@pytest.mark.gpu
+def test_plugin_runs_in_reference_scheduler(eng):
+    """Registered into the reference's own scheduler (compiled reference C),
+    a job set to "mi355x" returns the same outcomes as the reference's
+    "simulator" backend for the same job and seed (scheduler.c:201-236)."""
+    import json
+    import subprocess
+    import sys
+    from conftest import ROOT
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libmoonlab_ref.so")):
+        pytest.skip("oracle/_ref not built")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "plugin_host.py"), "run"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["rc"] == 0 and out["found"]
+    assert out["simulator"]["rc"] == 0 and out["mi355x"]["rc"] == 0
+    assert len(out["mi355x"]["outcomes"]) == 3000
+    assert out["mi355x"]["outcomes"] == out["simulator"]["outcomes"]
+
+
+def test_concurrent_jobs_match_sequential(eng):
+    """moonlab_scheduler_run from many threads at once (the reference's
+    tests/concurrency/conc_scheduler.c pattern): each job's outcomes equal
+    its outcomes run alone, for both precisions of execute_ex too."""
+    from concurrent.futures import ThreadPoolExecutor
+    from moonlab_amd import workloads
+    jobs = []
+    for i in range(12):
+        n = 8 + i % 9
+        j = eng.Job(n).set_num_shots(300).set_num_workers(1 + i % 3).set_rng_seed(7000 + i)
+        for row in workloads.random_layer_circuit(n, 6, 900 + i, h_layer=True):
+            t = int(row["type"])
+            j.add_gate(t, int(row["target"]), int(row["control"]), float(row["theta"]) if t in (7, 8, 9) else None)
+        jobs.append(j)
+    alone = [j.execute().outcomes for j in jobs]
+    alone_z = [j.execute_ex(precision=eng.C64, expect_z=True).expect_z for j in jobs]
+    with ThreadPoolExecutor(8) as ex:
+        mixed = list(ex.map(lambda j: j.execute().outcomes, jobs * 3))
+        mixed_z = list(ex.map(lambda j: j.execute_ex(precision=eng.C64, expect_z=True).expect_z, jobs * 2))
+    assert mixed == alone * 3
+    for a, b in zip(mixed_z, alone_z * 2):
+        assert _bytes_equal(a, b)
