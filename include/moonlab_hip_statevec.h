@@ -199,6 +199,13 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_correlation_zz(const moonlab_hip_st
 /* Addition: |a_i|^2 for all i into a host buffer of 2^n doubles. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_probabilities(const moonlab_hip_state_t *state,
                                                            double *out);
+/* Addition: <P> for the Pauli string with X on the qubits of x_mask only,
+ * Z on z_mask only and Y on both, in one pass (FP64 sums).  Replaces the
+ * exact path of vqe_measure_pauli_expectation (vqe.c:2156-2222), which clones
+ * the state, rotates it (H, S-dagger H) and sums parities. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_expect_pauli(const moonlab_hip_state_t *state,
+                                                          uint64_t x_mask, uint64_t z_mask,
+                                                          double *out);
 /* Addition: <a|b> (same n; either precision, products and sums in FP64, so a
  * complex64 run can be compared with a complex128 one without a copy). */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_inner_product(const moonlab_hip_state_t *a,

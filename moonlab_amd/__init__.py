@@ -295,6 +295,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_apply_gates_fused": ([vp, vp, ip, ip], ip),
         "moonlab_hip_fusion_plan": ([vp, ip, ip, u64p, ctypes.POINTER(ctypes.c_int), dp, ip], ip),
         "moonlab_hip_apply_gates_blocked": ([vp, vp, ip], ip),
+        "moonlab_hip_expect_pauli": ([vp, u64, u64, dp], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
         "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
@@ -719,6 +720,21 @@ class StateVector:
         v = ctypes.c_double()
         _check_hip(lib().moonlab_hip_expect_y(self._h, q, ctypes.byref(v)), "expect_y")
         return v.value
+
+    def expect_pauli(self, pauli: str) -> float:
+        """<P> for a Pauli string, pauli[q] in 'IXYZ' acting on qubit q (the
+        reference's pauli_term_t convention, vqe.c:2185-2199)."""
+        x = z = 0
+        for q, c in enumerate(pauli.upper()):
+            if c not in "IXYZ":
+                raise ValueError(f"bad Pauli letter {c!r}")
+            if c in "XY":
+                x |= 1 << q
+            if c in "YZ":
+                z |= 1 << q
+        out = ctypes.c_double()
+        _check_hip(lib().moonlab_hip_expect_pauli(self._h, x, z, ctypes.byref(out)), "expect_pauli")
+        return out.value
 
     def correlation_zz(self, q1: int, q2: int) -> float:
         v = ctypes.c_double()

@@ -63,6 +63,8 @@ hipError_t launch_probabilities(const SvDev &s, double *d_p);
 // Pauli-pair sums over one qubit: out[0] = <X_q>, out[1] = <Y_q>.
 hipError_t launch_xy(const SvDev &s, int q, double *d_partial, int blocks, double *d_out);
 // <Z_mask> = sum_i |a_i|^2 (-1)^popcount(i & mask).
+hipError_t launch_pauli(const SvDev &s, uint64_t x_mask, uint64_t z_mask, double *d_partial, int blocks,
+                        double *d_out);
 hipError_t launch_zmask(const SvDev &s, uint64_t mask, double *d_partial, int blocks,
                         double *d_out);
 // <a|b> (both same n / precision): out[0] = Re, out[1] = Im.

@@ -314,6 +314,29 @@ __global__ __launch_bounds__(256) void k_zmask(const cplx_t<R> *__restrict__ a, 
     block_store<1>(v, partial);
 }
 
+// <P> for P = i^ny X^x Z^z (Y = iXZ on the qubits in x & z; x != 0).  Pairs
+// (i, i ^ x) with bit p = lowest set bit of x clear read each amplitude once:
+// with w = a_i conj(a_j) and s = (-1)^popcount(i & z), the pair contributes
+// 2 s Re w (ny = 0 mod 4), -2 s Im w (1), -2 s Re w (2), 2 s Im w (3),
+// i.e. s (cr Re w + ci Im w).
+template <typename R>
+__global__ __launch_bounds__(256) void k_pauli(const cplx_t<R> *__restrict__ a, uint64_t npairs, int p,
+                                               uint64_t x, uint64_t z, double cr, double ci,
+                                               double *__restrict__ partial)
+{
+    double v[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += stride) {
+        const uint64_t i = ins0(k, p), j = i ^ x;
+        const double ar = a[i].x, ai = a[i].y, br = a[j].x, bi = a[j].y;
+        const double wr = fmar(ar, br, ai * bi);
+        const double wi = fmar(ai, br, -(ar * bi));
+        const double c = fmar(cr, wr, ci * wi);
+        v[0] += (__popcll(i & z) & 1) ? -c : c;
+    }
+    block_store<1>(v, partial);
+}
+
 template <typename R, typename R2>
 __global__ __launch_bounds__(256) void k_inner(const cplx_t<R> *__restrict__ a,
                                                const cplx_t<R2> *__restrict__ b, uint64_t dim,
@@ -414,6 +437,24 @@ hipError_t launch_xy(const SvDev &s, int q, double *d_partial, int blocks, doubl
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_reduce_final<<<1, 64, 0, s.stream>>>(d_partial, g, 2, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pauli(const SvDev &s, uint64_t x, uint64_t z, double *d_partial, int blocks, double *d_out)
+{
+    const int ny = __builtin_popcountll(x & z) & 3;
+    const double cr = ny == 0 ? 2.0 : ny == 2 ? -2.0 : 0.0;
+    const double ci = ny == 1 ? -2.0 : ny == 3 ? 2.0 : 0.0;
+    const int p = __builtin_ctzll(x);
+    const uint64_t np = (1ull << s.n) >> 1;
+    const int g = small_grid(np, blocks);
+    if (s.prec == MOONLAB_HIP_C64)
+        k_pauli<float><<<g, 256, 0, s.stream>>>((const float2 *)s.amps, np, p, x, z, cr, ci, d_partial);
+    else
+        k_pauli<double><<<g, 256, 0, s.stream>>>((const double2 *)s.amps, np, p, x, z, cr, ci, d_partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_reduce_final<<<1, 64, 0, s.stream>>>(d_partial, g, 1, d_out);
     return hipGetLastError();
 }
 

@@ -412,6 +412,24 @@ moonlab_hip_status_t moonlab_hip_correlation_zz(const moonlab_hip_state_t *cs, u
     return MOONLAB_HIP_OK;
 }
 
+moonlab_hip_status_t moonlab_hip_expect_pauli(const moonlab_hip_state_t *cs, uint64_t x_mask, uint64_t z_mask,
+                                              double *out)
+{
+    moonlab_hip_state_t *s = const_cast<moonlab_hip_state_t *>(cs);
+    if (!s || !out) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");
+    const uint64_t all = s->n >= 64 ? ~0ull : (1ull << s->n) - 1;
+    if ((x_mask | z_mask) & ~all) return fail(MOONLAB_HIP_ERR_INVALID_QUBIT, "Pauli mask names a qubit out of range");
+    moonlab_hip_status_t st = ensure_ws(s);
+    if (st) return st;
+    hipError_t e = x_mask ? launch_pauli(dev(s), x_mask, z_mask, s->ws.partial, 2048, s->ws.out)
+                          : launch_zmask(dev(s), z_mask, s->ws.partial, 2048, s->ws.out);
+    if (e != hipSuccess) return drv(e, "expect_pauli");
+    st = fetch_small(s, 1);
+    if (st) return st;
+    *out = s->h_out[0];
+    return MOONLAB_HIP_OK;
+}
+
 moonlab_hip_status_t moonlab_hip_inner_product(const moonlab_hip_state_t *a, const moonlab_hip_state_t *b,
                                                double *re, double *im)
 {

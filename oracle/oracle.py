@@ -75,6 +75,8 @@ class Oracle:
             getattr(L, f).restype = ctypes.c_double
         L.svo_correlation_zz.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
         L.svo_correlation_zz.restype = ctypes.c_double
+        L.svo_expect_pauli.argtypes = [ctypes.c_int, dp, ctypes.c_char_p]
+        L.svo_expect_pauli.restype = ctypes.c_double
         L.svo_norm_sq.argtypes = [ctypes.c_int, dp]
         L.svo_norm_sq.restype = ctypes.c_double
         L.svo_fidelity.argtypes = [ctypes.c_int, dp, dp]
@@ -139,6 +141,9 @@ class Oracle:
 
     def expect_y(self, n, amps, q):
         return self.L.svo_expect_y(n, _ptr(amps.view(np.float64)), q)
+
+    def expect_pauli(self, n, amps, pauli: str):
+        return self.L.svo_expect_pauli(n, _ptr(amps.view(np.float64)), pauli.encode())
 
     def correlation_zz(self, n, amps, q1, q2):
         return self.L.svo_correlation_zz(n, _ptr(amps.view(np.float64)), q1, q2)

@@ -484,6 +484,43 @@ double svo_expect_y(int n, const double *a, int q)
     return e;
 }
 
+double svo_expect_pauli(int n, const double *a, const char *pauli)
+{
+    /* vqe_measure_pauli_expectation's exact path (vqe.c:2156-2222): rotate
+     * a copy into the measurement basis -- H for X, S-dagger then H for Y
+     * (:2185-2199) -- then sum |psi'(x)|^2 (cabs squared) times the parity
+     * of x over the non-identity positions (:2214-2226). */
+    const uint64_t dim = 1ULL << n;
+    double *t = (double *)malloc(dim * 2 * sizeof(double));
+    if (!t) return 0.0;
+    memcpy(t, a, dim * 2 * sizeof(double));
+    uint64_t mask = 0;
+    for (int q = 0; q < n && pauli[q]; q++) {
+        svo_gate_t g = {0, q, -1, -1, 0.0, 0.0, 0.0};
+        switch (pauli[q]) {
+        case 'X':
+            g.type = SVO_H;
+            svo_apply_gate(n, t, &g);
+            break;
+        case 'Y':
+            g.type = SVO_SDG;
+            svo_apply_gate(n, t, &g);
+            g.type = SVO_H;
+            svo_apply_gate(n, t, &g);
+            break;
+        default: break;
+        }
+        if (pauli[q] == 'X' || pauli[q] == 'Y' || pauli[q] == 'Z') mask |= 1ULL << q;
+    }
+    double e = 0.0;
+    for (uint64_t x = 0; x < dim; x++) {
+        const double p = prob_cabs(t, x);
+        e += (__builtin_popcountll(x & mask) & 1) ? -p : p;
+    }
+    free(t);
+    return e;
+}
+
 double svo_correlation_zz(int n, const double *a, int q1, int q2)
 {
     /* P00 + P11 - P01 - P10 from four sequential sums

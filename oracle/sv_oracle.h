@@ -66,6 +66,9 @@ void   svo_expect_z_all(int n, const double *amps, double *z_out);
 double svo_expect_x(int n, const double *amps, int q);
 double svo_expect_y(int n, const double *amps, int q);
 double svo_correlation_zz(int n, const double *amps, int q1, int q2);
+/* <P> for a Pauli string (pauli[q] in "IXYZ" acts on qubit q), the exact
+ * path of vqe_measure_pauli_expectation (vqe.c:2156-2222). */
+double svo_expect_pauli(int n, const double *amps, const char *pauli);
 double svo_norm_sq(int n, const double *amps);
 double svo_fidelity(int n, const double *a, const double *b);
 

@@ -93,6 +93,26 @@ def test_state_copy_and_precision_conversion(eng, oracle):
         assert _bytes_equal(t.amplitudes(), a0.astype(np.complex64).astype(np.complex128))
 
 
+def test_pauli_expectation_vs_oracle(eng, oracle):
+    """moonlab_hip_expect_pauli against the restatement of
+    vqe_measure_pauli_expectation, every letter on every qubit position."""
+    rng = np.random.default_rng(32)
+    for n in (1, 2, 5, 9, 13):
+        gl = random_gate_list(n, 60, rng)
+        a = oracle.run(n, gl)
+        strings = ["".join(rng.choice(list("IXYZ"), size=n)) for _ in range(10)]
+        strings += ["Y" * n, "X" * n, "Z" * n, "I" * n]
+        with eng.StateVector(n) as s, eng.StateVector(n, eng.C64) as h:
+            s.load(a)
+            h.load(a)
+            for p in strings:
+                want = oracle.expect_pauli(n, a, p)
+                assert abs(s.expect_pauli(p) - want) < TOL, p
+                assert abs(h.expect_pauli(p) - want) < 1e-6, p
+            with pytest.raises(eng.EngineError):
+                s.expect_pauli("I" * n + "X")
+
+
 def test_mixed_precision_inner_product(eng):
     """<a|b> between a complex64 and a complex128 state: FP64 products of the
     stored values, within 1e-12 of numpy's."""

@@ -239,3 +239,42 @@ def test_splitmix_constant_pinned(oracle):
     x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & ((1 << 64) - 1)
     x = ((x ^ (x >> 27)) * 0x94D49BB133111EBB) & ((1 << 64) - 1)
     assert oracle.splitmix64(0x0123456789ABCDEF) == x ^ (x >> 31)
+
+
+def _pauli_dense(pauli):
+    """Kronecker product with qubit 0 as the least significant factor."""
+    P = {"I": np.eye(2), "X": np.array([[0, 1], [1, 0]]), "Y": np.array([[0, -1j], [1j, 0]]),
+         "Z": np.diag([1.0, -1.0])}
+    m = np.array([[1.0 + 0j]])
+    for c in pauli:               # pauli[q] acts on qubit q
+        m = np.kron(P[c], m)
+    return m
+
+
+def test_oracle_pauli_expectation_pinned(oracle, ref):
+    """The restatement of vqe_measure_pauli_expectation against brute force
+    (dense Kronecker products) and, for one- and two-letter strings, the
+    reference's own measurement_expectation_{x,y,z} / correlation_zz."""
+    rng = np.random.default_rng(31)
+    for n in (1, 2, 3, 5):
+        for _ in range(12):
+            gl = random_gate_list(n, 25, rng)
+            a = oracle.run(n, gl)
+            pauli = "".join(rng.choice(list("IXYZ"), size=n))
+            want = float(np.real(np.vdot(a, _pauli_dense(pauli) @ a)))
+            assert abs(oracle.expect_pauli(n, a, pauli) - want) < 1e-12, pauli
+    n = 6
+    gl = random_gate_list(n, 60, rng)
+    a = oracle.run(n, gl)
+    s = ref.state(n, gates_array([tuple(g) for g in gl]))
+    try:
+        for q in range(n):
+            for c, f in (("X", "measurement_expectation_x"), ("Y", "measurement_expectation_y"),
+                         ("Z", "measurement_expectation_z")):
+                p = "I" * q + c + "I" * (n - q - 1)
+                assert abs(oracle.expect_pauli(n, a, p) - getattr(ref.L, f)(s, q)) < 1e-12, p
+        for q in range(n - 1):
+            p = "I" * q + "ZZ" + "I" * (n - q - 2)
+            assert abs(oracle.expect_pauli(n, a, p) - ref.L.measurement_correlation_zz(s, q, q + 1)) < 1e-12
+    finally:
+        ref.free(s)
