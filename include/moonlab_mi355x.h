@@ -144,6 +144,18 @@ MOONLAB_API int moonlab_mi355x_hea_tfim_energies(moonlab_mi355x_batch_t *b,
                                                  int precision, double *energies,
                                                  moonlab_mi355x_batch_stats_t *stats);
 
+/* Energies and parameter-shift gradients of the same ansatz for many
+ * instances: each instance adds its base point and the +-pi/2 shift of each
+ * of its 2 n L angles to ONE batch run, and dE/dtheta_k =
+ * [E(theta + pi/2 e_k) - E(theta - pi/2 e_k)] / 2, exact for RY / RZ --
+ * the rule of vqe_compute_gradient's fallback (vqe.c:2094-2130); its adjoint
+ * fast path computes the same exact gradient.  gradients holds sum_i 2 n_i L_i
+ * values, instance after instance, in the angles' [layer][qubit][ry, rz] order. */
+MOONLAB_API int moonlab_mi355x_hea_tfim_gradients(moonlab_mi355x_batch_t *b,
+                                                  const moonlab_mi355x_hea_t *instances, int count,
+                                                  int precision, double *energies, double *gradients,
+                                                  moonlab_mi355x_batch_stats_t *stats);
+
 /* ---- Serving: continuous batching with HBM-footprint admission --------------
  *
  * A request loop in the spirit of the control plane's CIRCUIT / SHOTS verbs

@@ -268,6 +268,8 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_mi355x_response_free": ([ctypes.POINTER(_Response)], None),
         "moonlab_mi355x_server_stats": ([vp, ctypes.POINTER(_ServerStats)], ip),
         "moonlab_mi355x_hea_tfim_energies": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_hea_tfim_gradients": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, dp,
+                                               ctypes.POINTER(_BatchStats)], ip),
         # state-vector layer
         "moonlab_hip_device_count": ([], ip),
         "moonlab_hip_last_error": ([], ctypes.c_char_p),
@@ -896,6 +898,39 @@ class Batch:
         _check_sched(fn(self._h, items, len(circuits), ctypes.byref(st)), "batch_run")
         self.last_stats = self._stats(st)
         return [BatchResult(float(o[0][0]), o[1], o[2], o[3], o[4]) for o in outs]
+
+    @staticmethod
+    def _hea_array(instances):
+        arr = (_Hea * len(instances))()
+        keep = []
+        for i, (n, L, ang, h) in enumerate(instances):
+            a = np.ascontiguousarray(ang, dtype=np.float64).reshape(-1)
+            assert a.size == L * n * 2
+            keep.append(a)
+            arr[i].num_qubits = n
+            arr[i].num_layers = L
+            arr[i].angles = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            arr[i].h = h
+        return arr, keep
+
+    def hea_tfim_gradients(self, instances, precision: int = C128):
+        """Energies and parameter-shift gradients ([layers, n, 2] per instance),
+        every shifted circuit of every instance in one batch run."""
+        arr, keep = self._hea_array(instances)
+        e = np.zeros(len(instances))
+        sizes = [L * n * 2 for (n, L, _, _) in instances]
+        g = np.zeros(max(sum(sizes), 1))
+        st = _BatchStats()
+        _check_sched(lib().moonlab_mi355x_hea_tfim_gradients(
+            self._h, arr, len(instances), precision, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(st)), "hea_tfim_gradients")
+        self.last_stats = self._stats(st)
+        out, off = [], 0
+        for (n, L, _, _), k in zip(instances, sizes):
+            out.append(g[off:off + k].reshape(L, n, 2).copy())
+            off += k
+        del keep
+        return e, out
 
     def hea_tfim(self, instances, precision: int = C128) -> np.ndarray:
         """instances: (n, layers, angles[layers, n, 2], h) tuples -> energies."""

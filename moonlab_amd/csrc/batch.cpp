@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -602,6 +603,53 @@ int moonlab_mi355x_hea_tfim_energies(moonlab_mi355x_batch_t *b, const moonlab_mi
         double sx = 0.0;
         for (int q = 0; q < n; q++) sx += items[(size_t)i].expect_x[q];
         energies[i] = e - inst[i].h * sx;
+    }
+    return MOONLAB_SCHED_OK;
+}
+
+int moonlab_mi355x_hea_tfim_gradients(moonlab_mi355x_batch_t *b, const moonlab_mi355x_hea_t *inst, int count,
+                                      int precision, double *energies, double *gradients,
+                                      moonlab_mi355x_batch_stats_t *stats)
+{
+    if (!b || count < 0 || (count > 0 && (!inst || !energies || !gradients))) return MOONLAB_SCHED_BAD_ARG;
+    // instance i expands to its base point and +-pi/2 shifts of each of its
+    // P = 2 n L angles, all evaluated in one batch run
+    size_t total_inst = 0, total_angles = 0;
+    for (int i = 0; i < count; i++) {
+        const int n = inst[i].num_qubits, L = inst[i].num_layers;
+        if (n < 1 || n > kBatchMaxQubits || L < 0 || (L > 0 && !inst[i].angles)) return MOONLAB_SCHED_BAD_ARG;
+        const size_t P = (size_t)2 * n * L;
+        total_inst += 1 + 2 * P;
+        total_angles += (1 + 2 * P) * P;
+    }
+    std::vector<double> ang(total_angles);
+    std::vector<moonlab_mi355x_hea_t> ex(total_inst);
+    size_t ai = 0, ei = 0;
+    for (int i = 0; i < count; i++) {
+        const size_t P = (size_t)2 * inst[i].num_qubits * inst[i].num_layers;
+        for (size_t k = 0; k < 1 + 2 * P; k++) {
+            double *a = ang.data() + ai;
+            if (P) memcpy(a, inst[i].angles, P * sizeof(double));
+            if (k > 0) a[(k - 1) / 2] += ((k - 1) % 2 == 0 ? 1.0 : -1.0) * (M_PI / 2.0);
+            ex[ei] = inst[i];
+            ex[ei].angles = a;
+            ai += P;
+            ei++;
+        }
+    }
+    std::vector<double> e(total_inst);
+    const int rc = moonlab_mi355x_hea_tfim_energies(b, ex.data(), (int)total_inst, precision, e.data(), stats);
+    if (rc) return rc;
+    // dE/dtheta_k = [E(theta + pi/2 e_k) - E(theta - pi/2 e_k)] / 2, exact for
+    // RY / RZ (two-eigenvalue generators, vqe.c:2094-2100, 2119-2130)
+    ei = 0;
+    size_t gi = 0;
+    for (int i = 0; i < count; i++) {
+        const size_t P = (size_t)2 * inst[i].num_qubits * inst[i].num_layers;
+        energies[i] = e[ei];
+        for (size_t k = 0; k < P; k++) gradients[gi + k] = 0.5 * (e[ei + 1 + 2 * k] - e[ei + 2 + 2 * k]);
+        ei += 1 + 2 * P;
+        gi += P;
     }
     return MOONLAB_SCHED_OK;
 }

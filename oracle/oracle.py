@@ -181,6 +181,20 @@ class Oracle:
         return self.L.svo_hea_tfim_energy(n, layers, _ptr(angles), h,
                                           _ptr(scratch.view(np.float64)))
 
+    def hea_tfim_gradient(self, n, layers, angles, h):
+        """Two-term parameter shift over svo_hea_tfim_energy, the rule of
+        vqe_compute_gradient for the hardware-efficient ansatz (vqe.c:2094-2130)."""
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1).copy()
+        g = np.zeros_like(a)
+        for k in range(a.size):
+            a[k] += np.pi / 2
+            ep = self.hea_tfim_energy(n, layers, a, h)
+            a[k] -= np.pi
+            em = self.hea_tfim_energy(n, layers, a, h)
+            a[k] += np.pi / 2
+            g[k] = 0.5 * (ep - em)
+        return g.reshape(layers, n, 2)
+
     def compose_unitary(self, qubits, gates):
         g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
         k = len(qubits)

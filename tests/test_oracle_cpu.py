@@ -278,3 +278,23 @@ def test_oracle_pauli_expectation_pinned(oracle, ref):
             assert abs(oracle.expect_pauli(n, a, p) - ref.L.measurement_correlation_zz(s, q, q + 1)) < 1e-12
     finally:
         ref.free(s)
+
+
+def test_oracle_parameter_shift_gradient_pinned(oracle):
+    """The two-term shift rule on the HEA/TFIM energy equals the central
+    finite difference (a dropped factor, sign or index fails this)."""
+    rng = np.random.default_rng(33)
+    for n, L in ((2, 1), (3, 2), (5, 2)):
+        ang = rng.uniform(0, 2 * np.pi, size=(L, n, 2))
+        h = float(rng.uniform(0.5, 1.5))
+        g = oracle.hea_tfim_gradient(n, L, ang, h)
+        eps = 1e-5
+        fd = np.zeros_like(ang)
+        for idx in np.ndindex(ang.shape):
+            a = ang.copy()
+            a[idx] += eps
+            ep = oracle.hea_tfim_energy(n, L, a, h)
+            a[idx] -= 2 * eps
+            em = oracle.hea_tfim_energy(n, L, a, h)
+            fd[idx] = (ep - em) / (2 * eps)
+        assert np.max(np.abs(g - fd)) < 1e-8
