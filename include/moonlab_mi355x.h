@@ -113,6 +113,8 @@ typedef struct {
     int    tile_bits;
     int    total_passes;     /* HBM sweeps planned for the circuits above tile size */
     int    total_ops;
+    int    graph_replayed;   /* 1: the run replayed the HIP graph captured for this batch shape */
+    int    reserved;
 } moonlab_mi355x_batch_stats_t;
 
 MOONLAB_API moonlab_mi355x_batch_t *moonlab_mi355x_batch_create(void);
@@ -128,6 +130,14 @@ MOONLAB_API int moonlab_mi355x_batch_autotune(moonlab_mi355x_batch_t *b,
 MOONLAB_API int moonlab_mi355x_batch_run(moonlab_mi355x_batch_t *b,
                                          moonlab_mi355x_batch_item_t *items, int count,
                                          moonlab_mi355x_batch_stats_t *stats);
+/* HIP graphs (default on): a run's whole device step -- program upload,
+ * queue reset, the persistent launch, results download -- is captured when a
+ * batch shape (circuit sizes, passes, op and shot counts, launch
+ * configuration) repeats on the next run, and replayed with one
+ * hipGraphLaunch while it keeps repeating, as in a VQE loop that re-evaluates
+ * the same ansatz with new angles; a shape seen once runs without capture.
+ * device_seconds covers upload + launch + download. */
+MOONLAB_API int moonlab_mi355x_batch_set_graphs(moonlab_mi355x_batch_t *b, int enable);
 
 /* Hardware-efficient ansatz + open-chain transverse-field Ising energy, the
  * batched form of vqe_compute_energy: per layer RY, RZ on every qubit then

@@ -161,7 +161,8 @@ class _BatchItem(ctypes.Structure):
 class _BatchStats(ctypes.Structure):
     _fields_ = [("device_seconds", ctypes.c_double), ("wall_seconds", ctypes.c_double),
                 ("workgroups", ctypes.c_int), ("tile_bits", ctypes.c_int),
-                ("total_passes", ctypes.c_int), ("total_ops", ctypes.c_int)]
+                ("total_passes", ctypes.c_int), ("total_ops", ctypes.c_int),
+                ("graph_replayed", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class _Hea(ctypes.Structure):
@@ -851,6 +852,9 @@ class Batch:
     def __del__(self) -> None:
         self.free()
 
+    def set_graphs(self, enable: bool) -> None:
+        _check_sched(lib().moonlab_mi355x_batch_set_graphs(self._h, int(enable)), "batch_set_graphs")
+
     def set_config(self, workgroups_per_cu: int = 0, tile_bits: int = 0) -> None:
         _check_sched(lib().moonlab_mi355x_batch_set_config(self._h, workgroups_per_cu, tile_bits), "set_config")
 
@@ -858,7 +862,8 @@ class Batch:
     def _stats(st):
         return {"device_seconds": st.device_seconds, "wall_seconds": st.wall_seconds,
                 "workgroups": st.workgroups, "tile_bits": st.tile_bits,
-                "total_passes": st.total_passes, "total_ops": st.total_ops}
+                "total_passes": st.total_passes, "total_ops": st.total_ops,
+                "graph_replayed": bool(st.graph_replayed)}
 
     def run(self, circuits, autotune: bool = False) -> List[BatchResult]:
         """circuits: dicts with n, gates, precision (C128), observables (OBS_* bits),

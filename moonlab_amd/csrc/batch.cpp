@@ -264,6 +264,11 @@ struct moonlab_mi355x_batch {
     void       *d_scratch = nullptr;
     size_t      scratch_cap = 0;
     hipEvent_t  e0 = nullptr, e1 = nullptr;
+    // HIP graph of the last batch shape (see moonlab_mi355x_batch_set_graphs)
+    bool            use_graphs = true;
+    hipGraphExec_t  gexec = nullptr;
+    std::vector<uint64_t> gkey;       // shape of gexec
+    std::vector<uint64_t> last_key;   // shape of the previous run
 };
 
 namespace {
@@ -388,15 +393,61 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     double2 *scratch = (double2 *)b->d_scratch;
     double *tile_scratch = (double *)((char *)b->d_scratch + (size_t)grid * slot_amps * 16);
 
-    hipError_t e = hipMemcpyAsync(dp, hp, prog_bytes, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipEventRecord(b->e0, b->stream);
-    if (e == hipSuccess)
-        e = launch_batch((const BInst *)(dp + o_inst), (const int32_t *)(dp + o_order), count, b->d_counter,
-                         (const BPass *)(dp + o_pass), (const BOp *)(dp + o_ops), scratch, slot_amps, tile_scratch,
-                         tile_slot, (const double *)(dp + o_draw), (double *)(dr + r_obs),
-                         (uint64_t *)(dr + r_out), grid, b->stream);
+    // the device step: upload, queue reset + persistent launch, download
+    auto step = [&]() -> hipError_t {
+        hipError_t r = hipMemcpyAsync(dp, hp, prog_bytes, hipMemcpyHostToDevice, b->stream);
+        if (r == hipSuccess)
+            r = launch_batch((const BInst *)(dp + o_inst), (const int32_t *)(dp + o_order), count, b->d_counter,
+                             (const BPass *)(dp + o_pass), (const BOp *)(dp + o_ops), scratch, slot_amps,
+                             tile_scratch, tile_slot, (const double *)(dp + o_draw), (double *)(dr + r_obs),
+                             (uint64_t *)(dr + r_out), grid, b->stream);
+        if (r == hipSuccess) r = hipMemcpyAsync(b->h_res, dr, res_bytes, hipMemcpyDeviceToHost, b->stream);
+        return r;
+    };
+    // everything the captured nodes bake in: buffers, sizes, offsets, launch shape
+    const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)dp, (uint64_t)(uintptr_t)dr,
+                                       (uint64_t)(uintptr_t)hp, (uint64_t)(uintptr_t)b->h_res,
+                                       (uint64_t)(uintptr_t)b->d_scratch, prog_bytes, res_bytes, o_inst, o_order,
+                                       o_pass, o_ops, o_draw, r_obs, r_out, (uint64_t)count, (uint64_t)grid,
+                                       slot_amps, tile_slot};
+    // A shape seen for the first time runs eagerly; its second run is captured
+    // and launched as a graph; later runs of it replay that graph.
+    bool replayed = false;
+    hipError_t e = hipEventRecord(b->e0, b->stream);
+    if (e == hipSuccess && b->use_graphs && b->gexec && b->gkey == key) {
+        replayed = true;
+        e = hipGraphLaunch(b->gexec, b->stream);
+    } else if (e == hipSuccess && b->use_graphs && b->last_key == key) {
+        if (b->gexec) hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+        b->gkey.clear();
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+        if (ce == hipSuccess) {
+            const hipError_t se = step();
+            ce = hipStreamEndCapture(b->stream, &g);
+            if (se != hipSuccess) ce = se;
+        }
+        if (ce == hipSuccess) ce = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (ce == hipSuccess) {
+            b->gkey = key;
+            e = hipGraphLaunch(b->gexec, b->stream);
+        } else {
+            // this runtime cannot capture the step: run it eagerly from now on
+            fprintf(stderr, "moonlab-mi355x batch: graph capture failed (%s); launching directly\n",
+                    hipGetErrorString(ce));
+            (void)hipGetLastError();
+            if (b->gexec) hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            b->use_graphs = false;
+            e = step();
+        }
+    } else if (e == hipSuccess) {
+        e = step();
+    }
+    b->last_key = key;
     if (e == hipSuccess) e = hipEventRecord(b->e1, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->h_res, dr, res_bytes, hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
         fprintf(stderr, "moonlab-mi355x batch: %s\n", hipGetErrorString(e));
@@ -430,6 +481,8 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
             if (I.n > I.tile_bits) big_passes += I.num_passes;
         stats->total_passes = big_passes;
         stats->total_ops = (int)ops.size();
+        stats->graph_replayed = replayed ? 1 : 0;
+        stats->reserved = 0;
     }
     return MOONLAB_SCHED_OK;
 }
@@ -493,6 +546,19 @@ moonlab_mi355x_batch_t *moonlab_mi355x_batch_create(void)
     return b;
 }
 
+int moonlab_mi355x_batch_set_graphs(moonlab_mi355x_batch_t *b, int enable)
+{
+    if (!b) return MOONLAB_SCHED_BAD_ARG;
+    b->use_graphs = enable != 0;
+    if (!b->use_graphs && b->gexec) {
+        hipStreamSynchronize(b->stream);
+        hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+        b->gkey.clear();
+    }
+    return MOONLAB_SCHED_OK;
+}
+
 void moonlab_mi355x_batch_free(moonlab_mi355x_batch_t *b)
 {
     if (!b) return;
@@ -503,6 +569,7 @@ void moonlab_mi355x_batch_free(moonlab_mi355x_batch_t *b)
     hipFree(b->d_scratch);
     if (b->h_prog) hipHostFree(b->h_prog);
     if (b->h_res) hipHostFree(b->h_res);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->e0) hipEventDestroy(b->e0);
     if (b->e1) hipEventDestroy(b->e1);
     if (b->stream) hipStreamDestroy(b->stream);
