@@ -259,6 +259,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_mi355x_batch_create": ([], vp),
         "moonlab_mi355x_batch_free": ([vp], None),
         "moonlab_mi355x_batch_set_config": ([vp, ip, ip], ip),
+        "moonlab_mi355x_batch_set_graphs": ([vp, ip], ip),
         "moonlab_mi355x_batch_run": ([vp, ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(_BatchStats)], ip),
         "moonlab_mi355x_batch_autotune": ([vp, ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(_BatchStats)], ip),
         "moonlab_mi355x_server_create": ([ctypes.POINTER(_ServerConfig)], vp),

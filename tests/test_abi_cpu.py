@@ -43,6 +43,20 @@ def test_library_exports_every_declared_symbol():
     assert exported == names, sorted(exported ^ names)
 
 
+def test_binding_declares_every_symbol_it_calls():
+    """ctypes passes an undeclared argument as a C int, which truncates a
+    64-bit handle: every engine symbol the Python binding calls must carry
+    argtypes (the declaration table in moonlab_amd/__init__.py)."""
+    src = open(os.path.join(ROOT, "moonlab_amd", "__init__.py")).read()
+    called = set(re.findall(r"(?:lib\(\)|_lib|\bL)\.(moonlab_\w+)\(", src))
+    declared = set(re.findall(r'"(moonlab_\w+)": \(', src))
+    assert called, "no calls found"
+    assert called <= declared, sorted(called - declared)
+    L = m.lib()
+    for name in called:
+        assert getattr(L, name).argtypes is not None, name
+
+
 def test_job_introspection_and_errors():
     L = m.lib()
     assert not L.moonlab_job_create(0)
