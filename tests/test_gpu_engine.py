@@ -330,10 +330,41 @@ def test_execute_ex_outputs(eng, oracle):
     assert np.max(np.abs(rf.amplitudes - want)) < TOL
 
 
+def _dense_ref(a0, n, qs, u):
+    k = len(qs)
+    t = a0.reshape([2] * n)                      # axis i = qubit n-1-i
+    axes = [n - 1 - q for q in qs[::-1]]         # local bit j = qs[j]
+    moved = np.moveaxis(t, axes, list(range(k))).reshape(1 << k, -1)
+    return np.moveaxis((u @ moved).reshape([2] * n), list(range(k)), axes).reshape(-1)
+
+
+def test_dense_blocks_every_tile_width(eng):
+    """The matrix-core block kernel at every tile width it picks (64 groups
+    per tile on states just large enough, 128..512 groups on larger ones),
+    both precisions, low / scattered / high qubit sets, against numpy."""
+    import scipy.linalg
+    rng = np.random.default_rng(13)
+    for n in (9, 10, 11, 12, 13, 15):
+        for k in (3, 4, 5):
+            if n < k + 6:
+                continue
+            h = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+            u = scipy.linalg.expm(1j * (h + h.conj().T))
+            for qs in (list(range(k)), sorted(int(q) for q in rng.permutation(n)[:k]), list(range(n - k, n))):
+                a0 = random_state(n, rng)
+                want = _dense_ref(a0, n, qs, u)
+                with eng.StateVector(n) as s:
+                    got = s.load(a0).apply_dense(qs, u).amplitudes()
+                assert np.max(np.abs(got - want)) < TOL, (n, qs)
+                with eng.StateVector(n, eng.C64) as s:
+                    got = s.load(a0).apply_dense(qs, u).amplitudes()
+                assert np.max(np.abs(got - want)) < 2e-6, (n, qs, "c64")
+
+
 def test_fused_application_vs_unfused(eng, oracle):
     rng = np.random.default_rng(12)
     from moonlab_amd import workloads
-    for n, k in ((6, 3), (10, 4), (14, 5), (16, 5), (13, 2), (9, 1)):
+    for n, k in ((6, 3), (10, 4), (14, 5), (16, 5), (13, 2), (9, 1), (12, 3), (15, 4)):
         g = workloads.su2_cz_circuit(n, 6, 1000 + n)
         want = oracle.run(n, g)
         with eng.StateVector(n) as s:
