@@ -23,6 +23,8 @@
 #include <chrono>
 #include <climits>
 #include <complex>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "../../include/moonlab_mi355x.h"
@@ -311,23 +313,85 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     std::vector<std::pair<double, int>> cost((size_t)count);
     int64_t obs_total = 0, out_total = 0;
     int max_big = 0;
-    std::vector<GOp> gops;
+
+    // Lowering and pass planning are independent per circuit: chunks of
+    // circuits are planned on host threads into chunk-local pass / op lists,
+    // which are then concatenated in circuit order with their offsets shifted.
+    struct Chunk {
+        int                lo = 0, hi = 0;
+        std::vector<BPass> passes;
+        std::vector<BOp>   ops;
+        bool               bad = false;
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = count < 64 ? 1 : (int)std::min<unsigned>(16u, hw ? hw : 1u);
+    std::vector<Chunk> chunks((size_t)nthreads);
+    auto plan_chunk = [&](Chunk &c) {
+        std::vector<GOp> gops;
+        for (int i = c.lo; i < c.hi; i++) {
+            const moonlab_mi355x_batch_item_t &it = items[i];
+            const int n = it.num_qubits;
+            if (n < 1 || n > kBatchMaxQubits || (it.num_gates > 0 && !it.gates) || it.num_gates < 0 ||
+                it.num_shots < 0 || (it.num_shots > 0 && (!it.draws || !it.outcomes)) ||
+                (it.precision != MOONLAB_HIP_C128 && it.precision != MOONLAB_HIP_C64)) {
+                c.bad = true;
+                return;
+            }
+            gops.clear();
+            if (!lower(it.gates, it.num_gates, n, gops)) {
+                c.bad = true;
+                return;
+            }
+            BInst &I = insts[(size_t)i];
+            I.n = n;
+            I.prec = it.precision;
+            I.tile_bits = n < tbmax ? n : tbmax;
+            I.first_pass = (int)c.passes.size();   // chunk-local (passes and ops) until concatenation
+            plan_passes(gops, n, I.tile_bits, c.passes, c.ops);
+            I.num_passes = (int)c.passes.size() - I.first_pass;
+            // longest first: sweeps x amplitudes (resident circuits: ops x tile)
+            const double sweeps = n > I.tile_bits ? (double)(I.num_passes + 2) * 4.0 : 0.2 * (double)gops.size();
+            cost[(size_t)i] = {-(sweeps * (double)(1ull << n)), i};
+        }
+    };
+    for (int t = 0; t < nthreads; t++) {
+        chunks[(size_t)t].lo = (int)((int64_t)count * t / nthreads);
+        chunks[(size_t)t].hi = (int)((int64_t)count * (t + 1) / nthreads);
+    }
+    {
+        std::vector<std::thread> th;
+        int started = 0;
+        try {
+            for (int t = 1; t < nthreads; t++, started++) th.emplace_back(plan_chunk, std::ref(chunks[(size_t)t]));
+        } catch (const std::system_error &) {
+            // no more host threads: the calling thread plans what was not handed out
+        }
+        plan_chunk(chunks[0]);
+        for (int t = 1 + started; t < nthreads; t++) plan_chunk(chunks[(size_t)t]);
+        for (std::thread &x : th) x.join();
+    }
+    size_t total_passes = 0, total_ops = 0;
+    for (const Chunk &c : chunks) {
+        if (c.bad) return MOONLAB_SCHED_BAD_ARG;
+        total_passes += c.passes.size();
+        total_ops += c.ops.size();
+    }
+    passes.reserve(total_passes);
+    ops.reserve(total_ops);
+    for (const Chunk &c : chunks) {
+        const int pass_base = (int)passes.size(), op_base = (int)ops.size();
+        for (BPass P : c.passes) {
+            P.first_op += op_base;
+            passes.push_back(P);
+        }
+        ops.insert(ops.end(), c.ops.begin(), c.ops.end());
+        for (int i = c.lo; i < c.hi; i++) insts[(size_t)i].first_pass += pass_base;
+    }
+    // observables, draws and outcome slots in circuit order
     for (int i = 0; i < count; i++) {
         const moonlab_mi355x_batch_item_t &it = items[i];
-        const int n = it.num_qubits;
-        if (n < 1 || n > kBatchMaxQubits || (it.num_gates > 0 && !it.gates) || it.num_gates < 0 ||
-            it.num_shots < 0 || (it.num_shots > 0 && (!it.draws || !it.outcomes)) ||
-            (it.precision != MOONLAB_HIP_C128 && it.precision != MOONLAB_HIP_C64))
-            return MOONLAB_SCHED_BAD_ARG;
-        gops.clear();
-        if (!lower(it.gates, it.num_gates, n, gops)) return MOONLAB_SCHED_BAD_ARG;
         BInst &I = insts[(size_t)i];
-        I.n = n;
-        I.prec = it.precision;
-        I.tile_bits = n < tbmax ? n : tbmax;
-        I.first_pass = (int)passes.size();
-        plan_passes(gops, n, I.tile_bits, passes, ops);
-        I.num_passes = (int)passes.size() - I.first_pass;
+        const int n = I.n;
         I.flags = 0;
         if (it.observables & MOONLAB_MI355X_OBS_Z) I.flags |= BF_Z;
         if (it.observables & MOONLAB_MI355X_OBS_ZZ) I.flags |= BF_ZZ;
@@ -336,13 +400,10 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         I.obs_offset = obs_total;
         obs_total += obs_size(n);
         I.draw_offset = (int64_t)draws.size();
-        for (int s = 0; s < it.num_shots; s++) draws.push_back(it.draws[s]);
+        for (int sh = 0; sh < it.num_shots; sh++) draws.push_back(it.draws[sh]);
         I.out_offset = out_total;
         out_total += it.num_shots;
         if (n > I.tile_bits && n > max_big) max_big = n;
-        // longest first: sweeps x amplitudes (resident circuits: ops x tile)
-        const double sweeps = n > I.tile_bits ? (double)(I.num_passes + 2) * 4.0 : 0.2 * (double)gops.size();
-        cost[(size_t)i] = {-(sweeps * (double)(1ull << n)), i};
     }
     std::stable_sort(cost.begin(), cost.end());
     std::vector<int32_t> order((size_t)count);

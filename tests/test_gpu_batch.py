@@ -180,3 +180,26 @@ def test_graph_replays_match_direct_launches(eng, oracle):
         g_on.free()
         g_off.free()
     assert replayed == [False, False, True, True, True]
+
+
+def test_large_batch_matches_small_batches(eng, oracle):
+    """160 mixed circuits in one run (host planning split over threads) give
+    the same bits as the same circuits in runs of 40 (planned on one thread),
+    and match the oracle."""
+    rng = np.random.default_rng(36)
+    sizes = [int(x) for x in rng.integers(1, 17, size=160)]
+    circuits = _circuits(rng, sizes, ngates=40, shots=20)
+    b = eng.Batch()
+    try:
+        whole = b.run(circuits)
+        parts = []
+        for k in range(0, 160, 40):
+            parts += b.run(circuits[k:k + 40])
+    finally:
+        b.free()
+    for c, x, y in zip(circuits, whole, parts):
+        for f in ("expect_z", "zz_chain", "expect_x", "outcomes"):
+            assert getattr(x, f).tobytes() == getattr(y, f).tobytes(), (c["n"], f)
+    for c, r in list(zip(circuits, whole))[::16]:
+        a = oracle.run(c["n"], c["gates"])
+        assert np.max(np.abs(r.expect_z - oracle.expect_z_all(c["n"], a))) < TOL
