@@ -175,6 +175,15 @@ uint64_t needs_local(const GOp &o)
     return 0;
 }
 
+// Every qubit an op reads or writes (targets, diagonal qubits, controls).
+uint64_t touches(const GOp &o)
+{
+    uint64_t m = o.ctrl;
+    if (o.t1 >= 0) m |= 1ull << o.t1;
+    if (o.t2 >= 0) m |= 1ull << o.t2;
+    return m;
+}
+
 }  // namespace
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
@@ -223,15 +232,34 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass>
         cur.clear();
         S = anchor_mask;
     };
-    for (const GOp &o : gops) {
-        const uint64_t need = needs_local(o);
-        if ((S | need) != S) {
-            if (__builtin_popcountll(S | need) > (resident ? n : tb)) close();
-            S |= need;
-        }
-        cur.push_back(&o);
+    if (resident) {
+        for (const GOp &o : gops) cur.push_back(&o);
+        close();
+        return;
     }
-    close();
+    // Ops on disjoint qubits commute, so a pass takes, in circuit order, every
+    // op whose non-diagonal targets fit the tile set and that shares no qubit
+    // with an op already deferred from this pass; the deferred ops, still in
+    // circuit order, make up the rest of the circuit for the next pass.
+    std::vector<const GOp *> rest, later;
+    rest.reserve(gops.size());
+    for (const GOp &o : gops) rest.push_back(&o);
+    while (!rest.empty()) {
+        uint64_t blocked = 0;
+        later.clear();
+        for (const GOp *o : rest) {
+            const uint64_t t = touches(*o), need = needs_local(*o);
+            if (!(t & blocked) && __builtin_popcountll(S | need) <= tb) {
+                S |= need;
+                cur.push_back(o);
+            } else {
+                blocked |= t;
+                later.push_back(o);
+            }
+        }
+        close();
+        rest.swap(later);
+    }
 }
 
 namespace {

@@ -203,3 +203,23 @@ def test_large_batch_matches_small_batches(eng, oracle):
     for c, r in list(zip(circuits, whole))[::16]:
         a = oracle.run(c["n"], c["gates"])
         assert np.max(np.abs(r.expect_z - oracle.expect_z_all(c["n"], a))) < TOL
+
+
+def test_small_tiles_reorder_full_gate_table(eng, oracle):
+    """10-qubit tiles on 11..16-qubit circuits of every gate code: many
+    passes, ops moved across pass boundaries past ops on other qubits."""
+    rng = np.random.default_rng(37)
+    circuits = _circuits(rng, [11, 12, 13, 14, 15, 16], ngates=120, shots=50)
+    b = eng.Batch()
+    try:
+        b.set_config(tile_bits=10)
+        res = b.run(circuits)
+    finally:
+        b.free()
+    for c, r in zip(circuits, res):
+        a = oracle.run(c["n"], c["gates"])
+        assert abs(r.norm - 1.0) < TOL
+        assert np.max(np.abs(r.expect_z - oracle.expect_z_all(c["n"], a))) < TOL, c["n"]
+        xs = [oracle.expect_x(c["n"], a, q) for q in range(c["n"])]
+        assert np.max(np.abs(r.expect_x - xs)) < TOL, c["n"]
+        assert r.outcomes.tolist() == [oracle.measure_all_outcome(c["n"], a, u) for u in c["draws"]]
