@@ -174,6 +174,10 @@ MOONLAB_API int moonlab_hip_fusion_plan(const moonlab_hip_gate_t *gates, int num
 MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *state,
                                                                  const moonlab_hip_gate_t *gates,
                                                                  int num_gates);
+/* Addition: how many passes over the state moonlab_hip_apply_gates_blocked
+ * makes for this gate list on n qubits (host only, no device needed), or a
+ * negative moonlab_hip_status_t value. */
+MOONLAB_API int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *gates, int num_gates, int n);
 
 /* ---- Reductions ---------------------------------------------------------- */
 

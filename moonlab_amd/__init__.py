@@ -300,6 +300,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_fusion_plan": ([vp, ip, ip, u64p, ctypes.POINTER(ctypes.c_int), dp, ip], ip),
         "moonlab_hip_apply_gates_blocked": ([vp, vp, ip], ip),
         "moonlab_hip_expect_pauli": ([vp, u64, u64, dp], ip),
+        "moonlab_hip_blocked_pass_count": ([vp, ip, ip], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
         "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
@@ -814,6 +815,15 @@ def fusion_plan(gates, kmax: int = 5) -> List[FusedBlock]:
         m = mats[i, :2 * d * d].view(np.complex128).reshape(d, d).copy()
         out.append(FusedBlock(qs, int(counts[i]), m))
     return out
+
+
+def blocked_pass_count(gates, n: int) -> int:
+    """HBM passes StateVector.apply(gates, blocked=True) makes on n qubits."""
+    g = gate_array(gates)
+    k = lib().moonlab_hip_blocked_pass_count(g.ctypes.data, len(g), n)
+    if k < 0:
+        raise EngineError(f"blocked_pass_count failed ({k})")
+    return k
 
 
 def set_streaming_threshold(nbytes: int) -> None:

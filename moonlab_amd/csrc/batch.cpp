@@ -605,12 +605,25 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     hipError_t e = hipMallocAsync((void **)&d_ops, ops.size() * sizeof(BOp), st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // `ops` is a local vector
+    // 66 KiB complex128 tiles fit 2 workgroups per CU, 33 KiB complex64 tiles 4
+    const int prec = moonlab_hip_state_precision(s);
+    const int per_cu = prec == MOONLAB_HIP_C64 ? 4 : 2;
     for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
-        e = launch_tile_pass(moonlab_hip_state_device_ptr(s), moonlab_hip_state_precision(s), n, passes[i], d_ops,
-                             cus * 2, st);
+        e = launch_tile_pass(moonlab_hip_state_device_ptr(s), prec, n, passes[i], d_ops, cus * per_cu, st);
     if (d_ops) hipFreeAsync(d_ops, st);
     if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
     return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
+}
+
+int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n)
+{
+    if (ng < 0 || (ng > 0 && !g) || n < 1 || n > 40) return MOONLAB_HIP_ERR_BAD_ARG;
+    std::vector<GOp> gops;
+    if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
+    std::vector<BPass> passes;
+    std::vector<BOp> ops;
+    plan_passes(gops, n, n < kBatchMaxTileBits ? n : kBatchMaxTileBits, passes, ops);
+    return (int)passes.size();
 }
 
 moonlab_mi355x_batch_t *moonlab_mi355x_batch_create(void)

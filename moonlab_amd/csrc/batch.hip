@@ -23,6 +23,7 @@
 
 #include "batch_device.h"
 #include "sv_device.h"
+#include "sv_internal.h"
 
 namespace mlx {
 namespace {
@@ -454,10 +455,16 @@ __global__ void k_reset_counter(int *c) { *c = 0; }
 // the pass applied in LDS, then streamed back.
 constexpr int kPassThreads = 1024;
 
-template <typename R>
+// One cache-blocked pass over a large state: each workgroup streams tiles of
+// 2^P.nbits amplitudes through LDS and applies the pass's ops there.  The
+// next tile's loads are issued before the current tile's ops run, and the
+// barriers order LDS only, so loads, LDS work and the previous tile's stores
+// overlap.  NT: streaming accesses (runs are >= 16 amplitudes, whole lines).
+template <typename R, bool NT>
 __global__ __launch_bounds__(kPassThreads) void k_tile_pass(cplx_t<R> *__restrict__ a, BPass P,
                                                             const BOp *__restrict__ ops, uint64_t ntiles)
 {
+    constexpr int PER = (1 << kBatchMaxTileBits) / kPassThreads;
     __shared__ cplx_t<R> t[1 << kBatchMaxTileBits];
     __shared__ uint64_t dlo[64], dhi[64];
     const int tid = threadIdx.x;
@@ -473,16 +480,39 @@ __global__ __launch_bounds__(kPassThreads) void k_tile_pass(cplx_t<R> *__restric
         else dlo[j] = o;
     }
     __syncthreads();
-    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    // thread tid moves tile amplitudes l = tid + kPassThreads i (l % 64 = tid % 64)
+    const uint64_t lo = dlo[tid & 63];
+    const int hi0 = tid >> 6;
+    constexpr int HSTEP = kPassThreads / 64;
+    cplx_t<R> pre[PER];
+    uint64_t ti = blockIdx.x;
+    if (ti < ntiles) {
+        const uint64_t base = tile_base(ti, P) | lo;
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (tid + kPassThreads * i < T) pre[i] = ld<NT>(&a[base | dhi[hi0 + HSTEP * i]]);
+    }
+    for (; ti < ntiles; ti += gridDim.x) {
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (tid + kPassThreads * i < T) t[tid + kPassThreads * i] = pre[i];
+        lds_barrier();
+        const uint64_t nti = ti + gridDim.x;
+        if (nti < ntiles) {
+            const uint64_t nbase = tile_base(nti, P) | lo;
+#pragma unroll
+            for (int i = 0; i < PER; i++)
+                if (tid + kPassThreads * i < T) pre[i] = ld<NT>(&a[nbase | dhi[hi0 + HSTEP * i]]);
+        }
         const uint64_t base = tile_base(ti, P);
-        for (int l = tid; l < T; l += kPassThreads) t[l] = a[base | dlo[l & 63] | dhi[l >> 6]];
-        __syncthreads();
         for (int k = 0; k < P.num_ops; k++) {
             apply_op<R, kPassThreads>(t, T, ops[P.first_op + k], base);
-            __syncthreads();
+            lds_barrier();
         }
-        for (int l = tid; l < T; l += kPassThreads) a[base | dlo[l & 63] | dhi[l >> 6]] = t[l];
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (tid + kPassThreads * i < T) st<NT>(&a[(base | lo) | dhi[hi0 + HSTEP * i]], t[tid + kPassThreads * i]);
+        lds_barrier();   // every thread has read its write-back values before the next tile lands
     }
 }
 
@@ -493,10 +523,14 @@ hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const B
 {
     const uint64_t ntiles = 1ull << (n - P.nbits);
     const unsigned grid = (unsigned)(ntiles < (uint64_t)max_blocks ? ntiles : (uint64_t)max_blocks);
-    if (prec == 1)
-        k_tile_pass<float><<<grid, kPassThreads, 0, st>>>((float2 *)amps, P, d_ops, ntiles);
-    else
-        k_tile_pass<double><<<grid, kPassThreads, 0, st>>>((double2 *)amps, P, d_ops, ntiles);
+    const bool nt = ((1ull << n) * (prec == 1 ? 8ull : 16ull)) >= gate_tuning().nt_min_bytes;
+    if (prec == 1) {
+        if (nt) k_tile_pass<float, true><<<grid, kPassThreads, 0, st>>>((float2 *)amps, P, d_ops, ntiles);
+        else k_tile_pass<float, false><<<grid, kPassThreads, 0, st>>>((float2 *)amps, P, d_ops, ntiles);
+    } else {
+        if (nt) k_tile_pass<double, true><<<grid, kPassThreads, 0, st>>>((double2 *)amps, P, d_ops, ntiles);
+        else k_tile_pass<double, false><<<grid, kPassThreads, 0, st>>>((double2 *)amps, P, d_ops, ntiles);
+    }
     return hipGetLastError();
 }
 

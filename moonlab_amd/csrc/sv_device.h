@@ -32,6 +32,16 @@ __device__ __forceinline__ T cmul(const T &x, R pr, R pi)
     return y;
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every outstanding global load and store of the wave, which
+// drains a prefetch of the next tile and the write-back of the last one.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Streaming (non-temporal) loads and stores: a gate pass over a state larger
 // than the last-level cache has no reuse, and marking the traffic streaming
 // lets L2 and the MALL skip allocating it.

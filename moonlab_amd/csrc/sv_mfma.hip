@@ -130,16 +130,6 @@ constexpr int kThreads = 256;
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 5); }
 
-// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
-// waits for every outstanding global load and store of the wave, which would
-// drain the prefetch of the next tile and the write-back of the last one.
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Tile t's amplitude l lives at tile_base(t) | dlo[l % 64] | dhi[l / 64].
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const DenseGeom &G)
 {
