@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <climits>
@@ -594,12 +595,14 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     plan_passes(gops, n, tb, passes, ops);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
-    static int cus = 0;
+    static std::atomic<int> cached_cus{0};   // shared by every launching thread
+    int cus = cached_cus.load(std::memory_order_relaxed);
     if (!cus) {
         int dev = 0;
         hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (cus <= 0) cus = 256;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached_cus.store(cus, std::memory_order_relaxed);
     }
     BOp *d_ops = nullptr;
     hipError_t e = hipMallocAsync((void **)&d_ops, ops.size() * sizeof(BOp), st);

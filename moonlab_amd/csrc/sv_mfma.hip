@@ -28,6 +28,7 @@
 #include "sv_device.h"
 #include "sv_internal.h"
 
+#include <atomic>
 #include <vector>
 
 namespace mlx {
@@ -294,14 +295,18 @@ bool make_geom(int n, int k, int gb, const int *qubits, DenseGeom &G)
     return true;
 }
 
+// Cached launch facts are shared by every thread that launches; they are the
+// same whichever thread computes them first.
 int device_cus()
 {
-    static int cus = 0;
+    static std::atomic<int> cached{0};
+    int cus = cached.load(std::memory_order_relaxed);
     if (!cus) {
         int dev = 0;
         hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (cus <= 0) cus = 256;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached.store(cus, std::memory_order_relaxed);
     }
     return cus;
 }
@@ -311,16 +316,18 @@ int device_cus()
 template <typename R, int K, int GB, bool NT>
 int resident_blocks()
 {
-    static int cached = 0;   // one per kernel instantiation
-    if (!cached) {
+    static std::atomic<int> cached{0};   // one per kernel instantiation
+    int blocks = cached.load(std::memory_order_relaxed);
+    if (!blocks) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_mfma<R, K, GB, NT>, kThreads, 0) !=
                 hipSuccess ||
             per_cu < 1)
             per_cu = 1;
-        cached = per_cu * device_cus();
+        blocks = per_cu * device_cus();
+        cached.store(blocks, std::memory_order_relaxed);
     }
-    return cached;
+    return blocks;
 }
 
 template <typename R, int K, int GB>
