@@ -176,6 +176,43 @@ uint64_t needs_local(const GOp &o)
     return 0;
 }
 
+// Split a pass's tile-local ops into register segments (BOP_SEG): runs of at
+// least two ops whose non-diagonal targets fit kSegRegs tile bits, padded
+// with the highest unused tile bits (consecutive threads then touch
+// consecutive LDS slots).  Shorter runs stay single ops.
+void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vector<BOp> &out)
+{
+    if (seg_regs < 2 || nbits < seg_regs) {
+        out.insert(out.end(), in.begin(), in.end());
+        return;
+    }
+    auto need = [](const BOp &b) -> uint32_t {
+        if (b.kind == BOP_MAT1) return 1u << b.lq;
+        if (b.kind == BOP_SWAP) return (1u << b.lq) | (1u << b.lq2);
+        return 0u;
+    };
+    size_t i = 0;
+    while (i < in.size()) {
+        uint32_t regs = 0;
+        size_t j = i;
+        while (j < in.size() && __builtin_popcount(regs | need(in[j])) <= seg_regs) regs |= need(in[j++]);
+        if (j - i < 2) {
+            out.push_back(in[i++]);
+            continue;
+        }
+        for (int q = nbits - 1; q >= 0 && __builtin_popcount(regs) < seg_regs; q--) regs |= 1u << q;
+        BOp h{};
+        h.kind = BOP_SEG;
+        h.lq = (int32_t)(j - i);
+        h.lq2 = seg_regs;
+        for (int q = 0, k = 0; q < nbits; q++)
+            if (regs & (1u << q)) h.cl |= (uint32_t)q << (5 * k++);
+        out.push_back(h);
+        out.insert(out.end(), in.begin() + (long)i, in.begin() + (long)j);
+        i = j;
+    }
+}
+
 // Every qubit an op reads or writes (targets, diagonal qubits, controls).
 uint64_t touches(const GOp &o)
 {
@@ -188,11 +225,14 @@ uint64_t touches(const GOp &o)
 }  // namespace
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
-void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass> &passes,
+void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
                  std::vector<BOp> &ops)
 {
     const bool resident = n <= tb;
-    const int anchor = resident ? n : 4;   // low qubits kept in every tile: 256-byte runs at complex128
+    // low qubits kept in every tile (256-byte runs at complex128); at most
+    // tb - 2, so any op (two non-diagonal targets at most) fits an empty pass
+    // and every scan below schedules at least one op
+    const int anchor = resident ? n : (tb - 2 < 4 ? (tb - 2 > 0 ? tb - 2 : 0) : 4);
     uint64_t anchor_mask = anchor >= 64 ? ~0ull : ((1ull << anchor) - 1);
     std::vector<const GOp *> cur;
     uint64_t S = anchor_mask;
@@ -211,6 +251,8 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass>
                 local[q] = P.nbits;
                 P.qubits[P.nbits++] = (int8_t)q;
             }
+        std::vector<BOp> pass_ops;
+        pass_ops.reserve(cur.size());
         for (const GOp *o : cur) {
             BOp b{};
             b.kind = o->kind;
@@ -227,8 +269,10 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass>
                 b.lq2 = o->kind == BOP_SWAP ? local[o->t2] : INT32_MIN;
             }
             memcpy(b.m, o->m, sizeof b.m);
-            ops.push_back(b);
+            pass_ops.push_back(b);
         }
+        segment_ops(pass_ops, P.nbits, seg_regs, ops);
+        P.num_ops = (int)ops.size() - P.first_op;
         passes.push_back(P);
         cur.clear();
         S = anchor_mask;
@@ -376,7 +420,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
             I.prec = it.precision;
             I.tile_bits = n < tbmax ? n : tbmax;
             I.first_pass = (int)c.passes.size();   // chunk-local (passes and ops) until concatenation
-            plan_passes(gops, n, I.tile_bits, c.passes, c.ops);
+            plan_passes(gops, n, I.tile_bits, kSegRegsBatch, c.passes, c.ops);
             I.num_passes = (int)c.passes.size() - I.first_pass;
             // longest first: sweeps x amplitudes (resident circuits: ops x tile)
             const double sweeps = n > I.tile_bits ? (double)(I.num_passes + 2) * 4.0 : 0.2 * (double)gops.size();
@@ -592,7 +636,7 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     std::vector<BPass> passes;
     std::vector<BOp> ops;
     const int tb = n < kBatchMaxTileBits ? n : kBatchMaxTileBits;
-    plan_passes(gops, n, tb, passes, ops);
+    plan_passes(gops, n, tb, kSegRegsPass, passes, ops);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
     static std::atomic<int> cached_cus{0};   // shared by every launching thread
@@ -625,7 +669,7 @@ int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n)
     if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
     std::vector<BPass> passes;
     std::vector<BOp> ops;
-    plan_passes(gops, n, n < kBatchMaxTileBits ? n : kBatchMaxTileBits, passes, ops);
+    plan_passes(gops, n, n < kBatchMaxTileBits ? n : kBatchMaxTileBits, kSegRegsPass, passes, ops);
     return (int)passes.size();
 }
 

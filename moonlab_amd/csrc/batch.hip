@@ -122,6 +122,123 @@ __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
     }
 }
 
+// 2x2 on the register pair (j, j | 1 << P) of a segment, the arithmetic of
+// apply_op's BOP_MAT1 term by term.
+template <typename R, int NR, int P>
+__device__ __forceinline__ void seg_mat1(cplx_t<R> (&v)[1 << NR], const BOp &op, const int (&idx)[1 << NR])
+{
+    const R m0 = (R)op.m[0], m1 = (R)op.m[1], m2 = (R)op.m[2], m3 = (R)op.m[3];
+    const R m4 = (R)op.m[4], m5 = (R)op.m[5], m6 = (R)op.m[6], m7 = (R)op.m[7];
+#pragma unroll
+    for (int j = 0; j < (1 << NR); j++) {
+        if (j & (1 << P)) continue;
+        if ((idx[j] & (int)op.cl) != (int)op.cl) continue;
+        const cplx_t<R> a = v[j], b = v[j | (1 << P)];
+        cplx_t<R> x, y;
+        x.x = fmar(m0, a.x, -(m1 * a.y)) + fmar(m2, b.x, -(m3 * b.y));
+        x.y = fmar(m0, a.y, m1 * a.x) + fmar(m2, b.y, m3 * b.x);
+        y.x = fmar(m4, a.x, -(m5 * a.y)) + fmar(m6, b.x, -(m7 * b.y));
+        y.y = fmar(m4, a.y, m5 * a.x) + fmar(m6, b.y, m7 * b.x);
+        v[j] = x;
+        v[j | (1 << P)] = y;
+    }
+}
+
+// exchange of register bits P1 / P2 (apply_op's BOP_SWAP)
+template <typename R, int NR, int P1, int P2>
+__device__ __forceinline__ void seg_swap(cplx_t<R> (&v)[1 << NR], const BOp &op, const int (&idx)[1 << NR])
+{
+#pragma unroll
+    for (int j = 0; j < (1 << NR); j++) {
+        if (!(j & (1 << P1)) || (j & (1 << P2))) continue;
+        if ((idx[j] & (int)op.cl) != (int)op.cl) continue;
+        const int k = (j ^ (1 << P1)) | (1 << P2);
+        const cplx_t<R> x = v[j];
+        v[j] = v[k];
+        v[k] = x;
+    }
+}
+
+// A segment: nops ops whose non-diagonal targets lie in the NR local bits
+// rq.  Each thread gathers the 2^NR amplitudes of one group from LDS,
+// applies all ops in registers and writes them back: one LDS round trip and
+// one barrier for the whole segment instead of one per op.
+template <typename R, int STRIDE, int NR>
+__device__ __forceinline__ void apply_segment(cplx_t<R> *t, int T, const BOp *ops, int nops, uint32_t packed,
+                                              uint64_t base)
+{
+    int rq[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) rq[i] = (int)((packed >> (5 * i)) & 31u);
+    const int groups = T >> NR;
+    for (int g = threadIdx.x; g < groups; g += STRIDE) {
+        int gb = g;
+#pragma unroll
+        for (int i = 0; i < NR; i++) gb = (int)ins0((uint64_t)gb, rq[i]);
+        int idx[1 << NR];
+        cplx_t<R> v[1 << NR];
+#pragma unroll
+        for (int j = 0; j < (1 << NR); j++) {
+            idx[j] = gb;
+#pragma unroll
+            for (int i = 0; i < NR; i++)
+                if (j & (1 << i)) idx[j] |= 1 << rq[i];
+            v[j] = t[idx[j]];
+        }
+        for (int k = 0; k < nops; k++) {
+            const BOp &op = ops[k];
+            if ((base & op.cg) != op.cg) continue;   // a global control is 0 for this whole tile
+            if (op.kind == BOP_DIAG) {
+#pragma unroll
+                for (int j = 0; j < (1 << NR); j++) {
+                    if ((idx[j] & (int)op.cl) != (int)op.cl) continue;
+                    const int d = (bit_of(op.lq2, idx[j], base) << 1) | bit_of(op.lq, idx[j], base);
+                    v[j] = cmul(v[j], (R)op.m[2 * d], (R)op.m[2 * d + 1]);
+                }
+            } else if (op.kind == BOP_MAT1) {
+                const int p = op.lq == rq[0] ? 0 : (NR > 2 && op.lq != rq[1]) ? 2 : 1;
+                if (p == 0) seg_mat1<R, NR, 0>(v, op, idx);
+                else if (p == 1) seg_mat1<R, NR, 1>(v, op, idx);
+                else seg_mat1<R, NR, (NR > 2 ? 2 : 1)>(v, op, idx);
+            } else {   // BOP_SWAP of the register bits of lq and lq2
+                const int p1 = op.lq == rq[0] ? 0 : (NR > 2 && op.lq != rq[1]) ? 2 : 1;
+                const int p2 = op.lq2 == rq[0] ? 0 : (NR > 2 && op.lq2 != rq[1]) ? 2 : 1;
+                if (p1 == 0 && p2 == 1) seg_swap<R, NR, 0, 1>(v, op, idx);
+                else if (p1 == 1 && p2 == 0) seg_swap<R, NR, 1, 0>(v, op, idx);
+                else if (NR > 2) {
+                    constexpr int L = NR > 2 ? 2 : 1;
+                    if (p1 == 0 && p2 == 2) seg_swap<R, NR, 0, L>(v, op, idx);
+                    else if (p1 == 2 && p2 == 0) seg_swap<R, NR, L, 0>(v, op, idx);
+                    else if (p1 == 1 && p2 == 2) seg_swap<R, NR, 1, L>(v, op, idx);
+                    else if (p1 == 2 && p2 == 1) seg_swap<R, NR, L, 1>(v, op, idx);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < (1 << NR); j++) t[idx[j]] = v[j];
+    }
+}
+
+// Run a pass's op list on the tile in LDS: segment headers send their ops
+// through registers, other ops run one by one.  Every step ends with a
+// workgroup barrier (LDS only when the caller overlaps global traffic).
+template <typename R, int STRIDE, int NR, bool LDS_ONLY>
+__device__ __forceinline__ void run_pass_ops(cplx_t<R> *t, int T, const BOp *ops, int nops, uint64_t base)
+{
+    for (int k = 0; k < nops;) {
+        const BOp &op = ops[k];
+        if (op.kind == BOP_SEG) {
+            apply_segment<R, STRIDE, NR>(t, T, ops + k + 1, op.lq, op.cl, base);
+            k += 1 + op.lq;
+        } else {
+            apply_op<R, STRIDE>(t, T, op, base);
+            k++;
+        }
+        if (LDS_ONLY) lds_barrier();
+        else __syncthreads();
+    }
+}
+
 __device__ void build_deposit(Shared &sh, const BPass &p)
 {
     const int tid = threadIdx.x;
@@ -267,10 +384,7 @@ __device__ void run_circuit(Shared &sh, const BInst &I, const BPass *passes, con
                 for (int l = tid; l < T; l += NT) t[l] = slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]];
             }
             __syncthreads();
-            for (int k = 0; k < P.num_ops; k++) {
-                apply_op<R>(t, T, ops[P.first_op + k], base);
-                __syncthreads();
-            }
+            run_pass_ops<R, NT, kSegRegsBatch, false>(t, T, ops + P.first_op, P.num_ops, base);
             if (!resident) {
                 for (int l = tid; l < T; l += NT) slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]] = t[l];
                 __syncthreads();
@@ -505,10 +619,7 @@ __global__ __launch_bounds__(kPassThreads) void k_tile_pass(cplx_t<R> *__restric
                 if (tid + kPassThreads * i < T) pre[i] = ld<NT>(&a[nbase | dhi[hi0 + HSTEP * i]]);
         }
         const uint64_t base = tile_base(ti, P);
-        for (int k = 0; k < P.num_ops; k++) {
-            apply_op<R, kPassThreads>(t, T, ops[P.first_op + k], base);
-            lds_barrier();
-        }
+        run_pass_ops<R, kPassThreads, kSegRegsPass, true>(t, T, ops + P.first_op, P.num_ops, base);
 #pragma unroll
         for (int i = 0; i < PER; i++)
             if (tid + kPassThreads * i < T) st<NT>(&a[(base | lo) | dhi[hi0 + HSTEP * i]], t[tid + kPassThreads * i]);

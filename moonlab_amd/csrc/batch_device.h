@@ -13,7 +13,15 @@ enum BOpKind : int32_t {
     BOP_MAT1 = 0,   // 2x2 m[8] on local target lq, where controls are set
     BOP_DIAG = 1,   // diagonal: amplitude *= d[bit(qb) << 1 | bit(qa)], d = m[0..7]
     BOP_SWAP = 2,   // exchange local bits lq / lq2 where controls are set
+    BOP_SEG = 3,    // header: the next lq ops act only on the lq2 (2 or 3) local bits packed
+                    // 5 bits each (ascending) in cl, apart from diagonal and control bits;
+                    // they run on 2^lq2 amplitudes held in registers per thread
 };
+
+// Register bits of a segment: the single-state tile pass holds 8 amplitudes
+// per thread; the batch kernel, which also carries measurement state, 4.
+constexpr int kSegRegsPass = 3;
+constexpr int kSegRegsBatch = 2;
 
 struct BOp {
     int32_t  kind;

@@ -24,8 +24,9 @@ struct GOp {
 bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out);
 
 // Pack ops into passes over tile sets of `tb` qubits (n <= tb: one resident
-// pass over all n qubits) and convert qubits to tile-local indices.
-void plan_passes(const std::vector<GOp> &gops, int n, int tb, std::vector<BPass> &passes,
+// pass over all n qubits), convert qubits to tile-local indices and group
+// each pass's ops into register segments of `seg_regs` bits (BOP_SEG).
+void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
                  std::vector<BOp> &ops);
 
 }  // namespace mlx
