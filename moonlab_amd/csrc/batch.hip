@@ -82,6 +82,40 @@ __device__ __forceinline__ int bit_of(int32_t code, int l, uint64_t base)
     return code >= 0 ? (l >> code) & 1 : (int)((base >> (-1 - code)) & 1ull);
 }
 
+// The entries of a diagonal op are read with wave-uniform loads and the one
+// an amplitude needs is selected in registers (an index that differs per lane
+// would turn every read into a vector memory load).
+// readfirstlane moves a value every lane holds into a scalar register; a
+// select between such values stays a select instead of becoming a load from
+// a selected address (which would keep the entries in scratch memory).
+__device__ __forceinline__ float wave_uniform(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// entry k of four values held in registers
+template <typename R>
+__device__ __forceinline__ R pick4(int k, R a, R b, R c, R d)
+{
+    const R lo = (k & 1) ? b : a, hi = (k & 1) ? d : c;
+    return (k & 2) ? hi : lo;
+}
+
+// The four entries of a diagonal op (amplitude *= d[k]), wave-uniform.
+#define MLX_DIAG4(R, op)                                                                                  \
+    const R dr0 = wave_uniform((R)(op).m[0]), di0 = wave_uniform((R)(op).m[1]);                           \
+    const R dr1 = wave_uniform((R)(op).m[2]), di1 = wave_uniform((R)(op).m[3]);                           \
+    const R dr2 = wave_uniform((R)(op).m[4]), di2 = wave_uniform((R)(op).m[5]);                           \
+    const R dr3 = wave_uniform((R)(op).m[6]), di3 = wave_uniform((R)(op).m[7])
+
+
 template <typename R, int STRIDE = NT>
 __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
 {
@@ -104,10 +138,11 @@ __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
             t[i1] = y;
         }
     } else if (op.kind == BOP_DIAG) {
+        MLX_DIAG4(R, op);
         for (int l = tid; l < T; l += STRIDE) {
             if ((l & (int)op.cl) != (int)op.cl) continue;
             const int k = (bit_of(op.lq2, l, base) << 1) | bit_of(op.lq, l, base);
-            t[l] = cmul(t[l], (R)op.m[2 * k], (R)op.m[2 * k + 1]);
+            t[l] = cmul(t[l], pick4(k, dr0, dr1, dr2, dr3), pick4(k, di0, di1, di2, di3));
         }
     } else {   // BOP_SWAP
         const int b1 = 1 << op.lq, b2 = 1 << op.lq2;
@@ -189,11 +224,12 @@ __device__ __forceinline__ void apply_segment(cplx_t<R> *t, int T, const BOp *op
             const BOp &op = ops[k];
             if ((base & op.cg) != op.cg) continue;   // a global control is 0 for this whole tile
             if (op.kind == BOP_DIAG) {
+                MLX_DIAG4(R, op);
 #pragma unroll
                 for (int j = 0; j < (1 << NR); j++) {
                     if ((idx[j] & (int)op.cl) != (int)op.cl) continue;
                     const int d = (bit_of(op.lq2, idx[j], base) << 1) | bit_of(op.lq, idx[j], base);
-                    v[j] = cmul(v[j], (R)op.m[2 * d], (R)op.m[2 * d + 1]);
+                    v[j] = cmul(v[j], pick4(d, dr0, dr1, dr2, dr3), pick4(d, di0, di1, di2, di3));
                 }
             } else if (op.kind == BOP_MAT1) {
                 const int p = op.lq == rq[0] ? 0 : (NR > 2 && op.lq != rq[1]) ? 2 : 1;

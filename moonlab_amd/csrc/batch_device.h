@@ -20,7 +20,7 @@ enum BOpKind : int32_t {
 
 // Register bits of a segment: the single-state tile pass holds 8 amplitudes
 // per thread; the batch kernel, which also carries measurement state, 4.
-constexpr int kSegRegsPass = 3;
+constexpr int kSegRegsPass = 2;
 constexpr int kSegRegsBatch = 2;
 
 struct BOp {
