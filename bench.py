@@ -205,9 +205,19 @@ def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
     first = min(d.submit_time for d in done)
     last = max(d.finish_time for d in done)
     bad = sum(1 for d in done if d.status != 0)
+    by_path = {}
+    for name, p in (("batched", 0), ("large", 1), ("plain", 2)):
+        sel = [d for d in done if d.path == p]
+        if sel:
+            w = np.array([d.start_time - d.submit_time for d in sel])
+            sv = np.array([d.finish_time - d.start_time for d in sel])
+            by_path[name] = {"requests": len(sel), "wait_p50_ms": round(float(np.percentile(w, 50)) * 1e3, 2),
+                             "service_p50_ms": round(float(np.percentile(sv, 50)) * 1e3, 2),
+                             "service_sum_s": round(float(np.sum(sv)), 3)}
     return {"circuits_per_s": round(len(done) / (last - first), 2),
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "by_path": by_path,
             "makespan_s": round(last - first, 3), "failed": bad,
             "batches": int(st["batches"]), "largest_batch": int(st["largest_batch"]),
             "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2 ** 30, 3)}

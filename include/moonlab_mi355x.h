@@ -175,7 +175,7 @@ MOONLAB_API int moonlab_mi355x_hea_tfim_gradients(moonlab_mi355x_batch_t *b,
  * (at most batch_max_qubits) are co-batched into persistent batch launches,
  * each batch taking everything admitted since the previous one was launched
  * (two batch engines alternate, so a batch is built while another runs);
- * larger requests run cache-blocked on their own streams.  A request's
+ * larger requests run as fused blocks (or cache-blocked passes) on the large workers' streams.  A request's
  * results are the same alone and in any mix. */
 
 typedef struct moonlab_mi355x_server moonlab_mi355x_server_t;
@@ -185,8 +185,10 @@ typedef struct {
     int      batch_max_qubits;   /* co-batch requests up to this size (default 16, max 24) */
     int      max_batch;          /* requests per batch launch (default 1024) */
     int      batch_engines;      /* concurrent batch launches (default 2) */
-    int      large_workers;      /* concurrent large requests (default 4) */
+    int      large_workers;      /* concurrent large requests, each on its own stream (default 2) */
     int      sequential;         /* 1: plain dispatch -- one request at a time, one kernel per gate */
+    int      large_path;         /* large requests: 0 fused <= 5-qubit blocks on MFMA (default),
+                                    1 cache-blocked passes */
 } moonlab_mi355x_server_config_t;
 
 typedef struct {

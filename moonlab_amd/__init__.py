@@ -176,7 +176,7 @@ OBS_Z, OBS_ZZ, OBS_X = 1, 2, 4
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("batch_max_qubits", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("batch_engines", ctypes.c_int),
-                ("large_workers", ctypes.c_int), ("sequential", ctypes.c_int)]
+                ("large_workers", ctypes.c_int), ("sequential", ctypes.c_int), ("large_path", ctypes.c_int)]
 
 
 class _Request(ctypes.Structure):
@@ -988,9 +988,12 @@ class Server:
     """moonlab_mi355x_server_*: continuous batching with HBM admission."""
 
     def __init__(self, hbm_budget_bytes: int = 0, batch_max_qubits: int = 0, max_batch: int = 0,
-                 batch_engines: int = 0, large_workers: int = 0, sequential: bool = False) -> None:
+                 batch_engines: int = 0, large_workers: int = 0, sequential: bool = False,
+                 large_path: str = "fused") -> None:
+        if large_path not in ("fused", "blocked"):
+            raise ValueError("large_path is 'fused' or 'blocked'")
         cfg = _ServerConfig(hbm_budget_bytes, batch_max_qubits, max_batch, batch_engines, large_workers,
-                            int(sequential))
+                            int(sequential), 1 if large_path == "blocked" else 0)
         self._h = lib().moonlab_mi355x_server_create(ctypes.byref(cfg))
         if not self._h:
             raise EngineError("moonlab_mi355x_server_create failed (no HIP device?)")

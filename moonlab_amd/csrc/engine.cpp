@@ -7,6 +7,7 @@
 // 218-243); both are deterministic, so one simulation with the same per-worker
 // uniform streams gives the same outcomes.
 #include "engine.h"
+#include "sv_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -154,6 +155,8 @@ int run_circuit_gates(const RunRequest &rq, const moonlab_hip_gate_t *gl, int ng
     int st = acquire_state(rq.num_qubits, rq.precision, &s);
     if (st) return st;
     std::vector<moonlab_hip_gate_t> g(gl, gl + ng);
+    // a caller-chosen stream (a serving worker's) replaces the pooled state's own for this run
+    hipStream_t own = rq.stream ? state_bind_stream(s, (hipStream_t)rq.stream) : nullptr;
     hipStream_t stream = (hipStream_t)moonlab_hip_state_stream(s);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEventCreate(&e0);
@@ -189,6 +192,7 @@ int run_circuit_gates(const RunRequest &rq, const moonlab_hip_gate_t *gl, int ng
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    if (own) state_bind_stream(s, own);   // waits for the caller's stream, then hands the state back
     if (st) {
         moonlab_hip_synchronize(s);
         moonlab_hip_state_free(s);   // do not pool a state left in an unknown condition

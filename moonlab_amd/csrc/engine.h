@@ -47,6 +47,7 @@ struct RunRequest {
     bool                want_z;
     int                 fusion_max;     // 0 = unfused
     bool                blocked;        // cache-blocked tile passes
+    void               *stream = nullptr;   // run on this HIP stream instead of the state's own
 };
 
 struct RunResult {

@@ -14,13 +14,16 @@
 //     max_batch) into ONE persistent batch launch; with two engines, one batch
 //     is built and launched while the other runs, so arrivals never wait for
 //     a whole batch to drain before getting on the GPU;
-//   * large requests run cache-blocked on their own streams (large_workers in
-//     parallel), overlapping with the batches on the device;
+//   * large requests run as fused <= 5-qubit blocks on the matrix cores (or
+//     cache-blocked passes, large_path = 1), each large worker on one stream
+//     of its own for its whole life, overlapping with the batches on the
+//     device;
 //   * sequential mode is the plain baseline: one request at a time, one
 //     kernel per gate.
 // Results never depend on co-tenants: the batch engine's per-circuit planning
 // and reduction order, and the single-state kernels, are independent of what
 // else runs.
+#include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -152,16 +155,18 @@ void batch_thread(moonlab_mi355x_server_t *s, moonlab_mi355x_batch_t *eng)
     }
 }
 
-void run_single(Req *r, bool blocked)
+// mode 0: one kernel per gate; 1: cache-blocked passes; 2: fused <= 5-qubit blocks
+void run_single(Req *r, int mode, hipStream_t stream)
 {
-    std::vector<mlx::GateRec> none;
     mlx::RunRequest rq{};
     rq.num_qubits = r->n;
     rq.precision = r->prec;
     rq.draws = r->draws.empty() ? nullptr : r->draws.data();
     rq.num_draws = r->shots;
     rq.want_z = r->want_z;
-    rq.blocked = blocked;
+    rq.blocked = mode == 1;
+    rq.fusion_max = mode == 2 ? 5 : 0;
+    rq.stream = stream;
     mlx::RunResult rr;
     r->status = mlx::run_circuit_gates(rq, r->gates.data(), (int)r->gates.size(), rr);
     if (r->status == 0) {
@@ -175,6 +180,10 @@ void run_single(Req *r, bool blocked)
 
 void large_thread(moonlab_mi355x_server_t *s)
 {
+    // one stream per worker for its whole life: a request's kernels queue
+    // behind its own worker's only, not behind every pooled state's stream
+    hipStream_t stream = nullptr;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
     for (;;) {
         Req *r = nullptr;
         {
@@ -185,14 +194,14 @@ void large_thread(moonlab_mi355x_server_t *s)
                 const Req *h = s->large_q.front();
                 return s->admitted_bytes == 0 || s->admitted_bytes + h->footprint <= s->cfg.hbm_budget_bytes;
             });
-            if (s->large_q.empty()) return;
+            if (s->large_q.empty()) break;
             std::vector<Req *> got = admit(s, s->large_q, 1);
             if (got.empty()) continue;
             r = got[0];
             r->t_start = clock_s() - s->t0;
             s->stats.large_requests++;
         }
-        run_single(r, true);
+        run_single(r, s->cfg.large_path == 1 ? 1 : 2, stream);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             r->path = 1;
@@ -201,6 +210,7 @@ void large_thread(moonlab_mi355x_server_t *s)
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
+    if (stream) hipStreamDestroy(stream);
 }
 
 void sequential_thread(moonlab_mi355x_server_t *s)
@@ -217,7 +227,7 @@ void sequential_thread(moonlab_mi355x_server_t *s)
             if (s->admitted_bytes > s->stats.peak_bytes_admitted) s->stats.peak_bytes_admitted = s->admitted_bytes;
             r->t_start = clock_s() - s->t0;
         }
-        run_single(r, false);
+        run_single(r, 0, nullptr);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             r->path = 2;
@@ -240,13 +250,14 @@ moonlab_mi355x_server_t *moonlab_mi355x_server_create(const moonlab_mi355x_serve
     s->cfg.batch_max_qubits = 16;
     s->cfg.max_batch = 1024;
     s->cfg.batch_engines = 2;
-    s->cfg.large_workers = 4;
+    s->cfg.large_workers = 2;
     if (cfg) {
         if (cfg->hbm_budget_bytes) s->cfg.hbm_budget_bytes = cfg->hbm_budget_bytes;
         if (cfg->batch_max_qubits > 0) s->cfg.batch_max_qubits = cfg->batch_max_qubits;
         if (cfg->max_batch > 0) s->cfg.max_batch = cfg->max_batch;
         if (cfg->batch_engines > 0) s->cfg.batch_engines = cfg->batch_engines;
         if (cfg->large_workers > 0) s->cfg.large_workers = cfg->large_workers;
+        s->cfg.large_path = cfg->large_path == 1 ? 1 : 0;
         s->cfg.sequential = cfg->sequential;
     }
     if (s->cfg.batch_max_qubits > MOONLAB_MI355X_BATCH_MAX_QUBITS) s->cfg.batch_max_qubits = MOONLAB_MI355X_BATCH_MAX_QUBITS;

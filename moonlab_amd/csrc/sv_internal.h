@@ -28,6 +28,10 @@ struct GateTuning {
 };
 GateTuning &gate_tuning();
 
+// Point a state's work at another stream (waits for the state's current
+// stream first); returns the stream it used before.
+hipStream_t state_bind_stream(moonlab_hip_state_t *s, hipStream_t st);
+
 // Gate launchers (sv_gates.hip).  All return a hipError_t value.
 hipError_t launch_gate(const SvDev &s, const moonlab_hip_gate_t &g);
 hipError_t launch_matrix_1q(const SvDev &s, int q, const double m[8]);

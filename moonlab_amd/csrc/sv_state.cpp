@@ -136,6 +136,20 @@ moonlab_hip_status_t check_gate(const moonlab_hip_state_t *s, const moonlab_hip_
 
 }  // namespace
 
+namespace mlx {
+
+hipStream_t state_bind_stream(moonlab_hip_state_t *s, hipStream_t st)
+{
+    hipStream_t old = s->stream;
+    if (old != st) {
+        hipStreamSynchronize(old);
+        s->stream = st;
+    }
+    return old;
+}
+
+}  // namespace mlx
+
 extern "C" {
 
 int moonlab_hip_device_count(void)

@@ -19,11 +19,12 @@ def _submit_all(srv, reqs):
             for r in reqs]
 
 
-def test_served_results_match_oracle(eng, oracle):
+@pytest.mark.parametrize("large_path", ["fused", "blocked"])
+def test_served_results_match_oracle(eng, oracle, large_path):
     reqs = workloads.serving_requests(40, 7, 1000.0, n_range=(6, 14), depth_range=(4, 12), shots=256)
     for r in reqs:
         r["precision"] = eng.C128
-    with eng.Server(batch_max_qubits=12) as srv:
+    with eng.Server(batch_max_qubits=12, large_path=large_path) as srv:
         tickets = _submit_all(srv, reqs)
         res = [srv.wait(t) for t in tickets]
     paths = set()
