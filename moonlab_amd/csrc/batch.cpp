@@ -106,6 +106,17 @@ void diag2_op(std::vector<GOp> &out, int qa, int qb, uint64_t ctrl, cd d0, cd d1
 }  // namespace
 
 // Steps 1 and 2: gate list -> global ops.
+// X on t where every control in ctrl is 1: a pure exchange of pair values.
+void xperm_op(std::vector<GOp> &out, int t, uint64_t ctrl)
+{
+    GOp o{};
+    o.kind = BOP_XPERM;
+    o.t1 = t;
+    o.t2 = -1;
+    o.ctrl = ctrl;
+    out.push_back(o);
+}
+
 bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out)
 {
     std::vector<std::array<cd, 4>> pend((size_t)n);
@@ -139,12 +150,12 @@ bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out)
         const int c = x.control, t = x.target, e = x.extra;
         const double co = cos(x.theta / 2.0), si = sin(x.theta / 2.0);
         switch (x.type) {
-        case MOONLAB_HIP_GATE_CNOT: mat_op(out, t, 1ull << c, 0, 1, 1, 0); break;
+        case MOONLAB_HIP_GATE_CNOT: xperm_op(out, t, 1ull << c); break;
         case MOONLAB_HIP_GATE_CY: mat_op(out, t, 1ull << c, 0, I, -I, 0); break;
         case MOONLAB_HIP_GATE_CY_TRUE: mat_op(out, t, 1ull << c, 0, -I, I, 0); break;
         case MOONLAB_HIP_GATE_CRX: mat_op(out, t, 1ull << c, co, cd(0, -si), cd(0, -si), co); break;
         case MOONLAB_HIP_GATE_CRY: mat_op(out, t, 1ull << c, co, -si, si, co); break;
-        case MOONLAB_HIP_GATE_TOFFOLI: mat_op(out, t, (1ull << c) | (1ull << e), 0, 1, 1, 0); break;
+        case MOONLAB_HIP_GATE_TOFFOLI: xperm_op(out, t, (1ull << c) | (1ull << e)); break;
         case MOONLAB_HIP_GATE_CZ: diag2_op(out, c, t, 0, 1, 1, 1, -1); break;
         case MOONLAB_HIP_GATE_CPHASE: diag2_op(out, c, t, 0, 1, 1, 1, cd(cos(x.theta), sin(x.theta))); break;
         case MOONLAB_HIP_GATE_CRZ:   // index = bit(c) << 1 | bit(t)
@@ -171,7 +182,7 @@ namespace {
 
 uint64_t needs_local(const GOp &o)
 {
-    if (o.kind == BOP_MAT1) return 1ull << o.t1;
+    if (o.kind == BOP_MAT1 || o.kind == BOP_XPERM) return 1ull << o.t1;
     if (o.kind == BOP_SWAP) return (1ull << o.t1) | (1ull << o.t2);
     return 0;
 }
@@ -187,7 +198,7 @@ void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vecto
         return;
     }
     auto need = [](const BOp &b) -> uint32_t {
-        if (b.kind == BOP_MAT1) return 1u << b.lq;
+        if (b.kind == BOP_MAT1 || b.kind == BOP_XPERM) return 1u << b.lq;
         if (b.kind == BOP_SWAP) return (1u << b.lq) | (1u << b.lq2);
         return 0u;
     };

@@ -137,6 +137,15 @@ __device__ void apply_op(cplx_t<R> *t, int T, const BOp &op, uint64_t base)
             t[i0] = x;
             t[i1] = y;
         }
+    } else if (op.kind == BOP_XPERM) {
+        const int half = T >> 1;
+        for (int p = tid; p < half; p += STRIDE) {
+            const int i0 = (int)ins0((uint64_t)p, op.lq), i1 = i0 | (1 << op.lq);
+            if ((i0 & (int)op.cl) != (int)op.cl) continue;
+            const cplx_t<R> a = t[i0];
+            t[i0] = t[i1];
+            t[i1] = a;
+        }
     } else if (op.kind == BOP_DIAG) {
         MLX_DIAG4(R, op);
         for (int l = tid; l < T; l += STRIDE) {
@@ -176,6 +185,20 @@ __device__ __forceinline__ void seg_mat1(cplx_t<R> (&v)[1 << NR], const BOp &op,
         y.y = fmar(m4, a.y, m5 * a.x) + fmar(m6, b.y, m7 * b.x);
         v[j] = x;
         v[j | (1 << P)] = y;
+    }
+}
+
+// controlled X on register bit P (apply_op's BOP_XPERM)
+template <typename R, int NR, int P>
+__device__ __forceinline__ void seg_xperm(cplx_t<R> (&v)[1 << NR], const BOp &op, const int (&idx)[1 << NR])
+{
+#pragma unroll
+    for (int j = 0; j < (1 << NR); j++) {
+        if (j & (1 << P)) continue;
+        if ((idx[j] & (int)op.cl) != (int)op.cl) continue;
+        const cplx_t<R> a = v[j];
+        v[j] = v[j | (1 << P)];
+        v[j | (1 << P)] = a;
     }
 }
 
@@ -236,6 +259,11 @@ __device__ __forceinline__ void apply_segment(cplx_t<R> *t, int T, const BOp *op
                 if (p == 0) seg_mat1<R, NR, 0>(v, op, idx);
                 else if (p == 1) seg_mat1<R, NR, 1>(v, op, idx);
                 else seg_mat1<R, NR, (NR > 2 ? 2 : 1)>(v, op, idx);
+            } else if (op.kind == BOP_XPERM) {
+                const int p = op.lq == rq[0] ? 0 : (NR > 2 && op.lq != rq[1]) ? 2 : 1;
+                if (p == 0) seg_xperm<R, NR, 0>(v, op, idx);
+                else if (p == 1) seg_xperm<R, NR, 1>(v, op, idx);
+                else seg_xperm<R, NR, (NR > 2 ? 2 : 1)>(v, op, idx);
             } else {   // BOP_SWAP of the register bits of lq and lq2
                 const int p1 = op.lq == rq[0] ? 0 : (NR > 2 && op.lq != rq[1]) ? 2 : 1;
                 const int p2 = op.lq2 == rq[0] ? 0 : (NR > 2 && op.lq2 != rq[1]) ? 2 : 1;
@@ -603,7 +631,7 @@ __global__ void k_reset_counter(int *c) { *c = 0; }
 // One cache-blocked pass over a large state: blocks walk the tiles of the
 // pass's tile set (grid-stride), each tile streamed HBM -> LDS, every op of
 // the pass applied in LDS, then streamed back.
-constexpr int kPassThreads = 1024;
+constexpr int kPassThreads = 512;   // two (complex128) or four (complex64) workgroups per CU overlap their phases
 
 // One cache-blocked pass over a large state: each workgroup streams tiles of
 // 2^P.nbits amplitudes through LDS and applies the pass's ops there.  The

@@ -13,6 +13,7 @@ enum BOpKind : int32_t {
     BOP_MAT1 = 0,   // 2x2 m[8] on local target lq, where controls are set
     BOP_DIAG = 1,   // diagonal: amplitude *= d[bit(qb) << 1 | bit(qa)], d = m[0..7]
     BOP_SWAP = 2,   // exchange local bits lq / lq2 where controls are set
+    BOP_XPERM = 4,  // controlled X: exchange the pair across local bit lq where controls are set
     BOP_SEG = 3,    // header: the next lq ops act only on the lq2 (2 or 3) local bits packed
                     // 5 bits each (ascending) in cl, apart from diagonal and control bits;
                     // they run on 2^lq2 amplitudes held in registers per thread

@@ -26,6 +26,8 @@ static void apply_op(std::vector<cd> &t, int T, const mlx::BOp &op, uint64_t bas
     if ((base & op.cg) != op.cg) return;
     if (op.kind == mlx::BOP_MAT1) {
         for (int p = 0; p < T / 2; p++) { int i0 = (int)ins0(p, op.lq), i1 = i0 | (1 << op.lq); if ((i0 & (int)op.cl) != (int)op.cl) continue; mat1(t[i0], t[i1], op); }
+    } else if (op.kind == mlx::BOP_XPERM) {
+        for (int p = 0; p < T / 2; p++) { int i0 = (int)ins0(p, op.lq), i1 = i0 | (1 << op.lq); if ((i0 & (int)op.cl) != (int)op.cl) continue; std::swap(t[i0], t[i1]); }
     } else if (op.kind == mlx::BOP_DIAG) {
         for (int l = 0; l < T; l++) { if ((l & (int)op.cl) != (int)op.cl) continue; int k = (bit_of(op.lq2, l, base) << 1) | bit_of(op.lq, l, base); t[l] *= cd(op.m[2 * k], op.m[2 * k + 1]); }
     } else {
@@ -45,10 +47,15 @@ static void apply_segment(std::vector<cd> &t, int T, const mlx::BOp *ops, int no
             if ((base & op.cg) != op.cg) continue;
             if (op.kind == mlx::BOP_DIAG) {
                 for (int j = 0; j < (1 << NR); j++) { if ((idx[j] & (int)op.cl) != (int)op.cl) continue; int d = (bit_of(op.lq2, idx[j], base) << 1) | bit_of(op.lq, idx[j], base); v[j] *= cd(op.m[2 * d], op.m[2 * d + 1]); }
-            } else if (op.kind == mlx::BOP_MAT1) {
+            } else if (op.kind == mlx::BOP_MAT1 || op.kind == mlx::BOP_XPERM) {
                 int p = -1; for (int i = 0; i < NR; i++) if (rq[i] == op.lq) p = i;
-                if (p < 0) { printf("MAT1 target not in registers\n"); exit(1); }
-                for (int j = 0; j < (1 << NR); j++) { if (j & (1 << p)) continue; if ((idx[j] & (int)op.cl) != (int)op.cl) continue; mat1(v[j], v[j | (1 << p)], op); }
+                if (p < 0) { printf("target not in registers\n"); exit(1); }
+                for (int j = 0; j < (1 << NR); j++) {
+                    if (j & (1 << p)) continue;
+                    if ((idx[j] & (int)op.cl) != (int)op.cl) continue;
+                    if (op.kind == mlx::BOP_MAT1) mat1(v[j], v[j | (1 << p)], op);
+                    else std::swap(v[j], v[j | (1 << p)]);
+                }
             } else {
                 int p1 = -1, p2 = -1; for (int i = 0; i < NR; i++) { if (rq[i] == op.lq) p1 = i; if (rq[i] == op.lq2) p2 = i; }
                 if (p1 < 0 || p2 < 0) { printf("SWAP target not in registers\n"); exit(1); }
