@@ -178,6 +178,10 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_sta
  * makes for this gate list on n qubits (host only, no device needed), or a
  * negative moonlab_hip_status_t value. */
 MOONLAB_API int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *gates, int num_gates, int n);
+/* Addition: LDS tile (11 or 12 qubits) and workgroup size (256 or 512) of
+ * the blocked path; 0 keeps a value.  Process-wide, like the gate tuning.
+ * Results do not depend on it (the same ops in the same order per amplitude). */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads);
 
 /* ---- Reductions ---------------------------------------------------------- */
 

@@ -301,6 +301,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_apply_gates_blocked": ([vp, vp, ip], ip),
         "moonlab_hip_expect_pauli": ([vp, u64, u64, dp], ip),
         "moonlab_hip_blocked_pass_count": ([vp, ip, ip], ip),
+        "moonlab_hip_set_blocked_tuning": ([ip, ip], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
         "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
@@ -824,6 +825,12 @@ def blocked_pass_count(gates, n: int) -> int:
     if k < 0:
         raise EngineError(f"blocked_pass_count failed ({k})")
     return k
+
+
+def set_blocked_tuning(tile_bits: int = 0, threads: int = 0) -> None:
+    """LDS tile (11 or 12 qubits) and workgroup size (256 or 512) of the
+    cache-blocked path; 0 keeps a value."""
+    _check_hip(lib().moonlab_hip_set_blocked_tuning(tile_bits, threads), "set_blocked_tuning")
 
 
 def set_streaming_threshold(nbytes: int) -> None:

@@ -32,11 +32,12 @@
 #include "batch_device.h"
 #include "batch_plan.h"
 #include "fusion.h"
+#include "sv_internal.h"
 
 namespace mlx {
 
-hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int max_blocks,
-                            hipStream_t st);
+hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int threads,
+                            int tile_bits, hipStream_t st);
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
@@ -646,7 +647,8 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
     std::vector<BPass> passes;
     std::vector<BOp> ops;
-    const int tb = n < kBatchMaxTileBits ? n : kBatchMaxTileBits;
+    const GateTuning tune = gate_tuning();
+    const int tb = n < tune.blocked_tile_bits ? n : tune.blocked_tile_bits;
     plan_passes(gops, n, tb, kSegRegsPass, passes, ops);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
@@ -663,11 +665,10 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     hipError_t e = hipMallocAsync((void **)&d_ops, ops.size() * sizeof(BOp), st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // `ops` is a local vector
-    // 66 KiB complex128 tiles fit 2 workgroups per CU, 33 KiB complex64 tiles 4
     const int prec = moonlab_hip_state_precision(s);
-    const int per_cu = prec == MOONLAB_HIP_C64 ? 4 : 2;
     for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
-        e = launch_tile_pass(moonlab_hip_state_device_ptr(s), prec, n, passes[i], d_ops, cus * per_cu, st);
+        e = launch_tile_pass(moonlab_hip_state_device_ptr(s), prec, n, passes[i], d_ops, cus, tune.blocked_threads,
+                             tune.blocked_tile_bits, st);
     if (d_ops) hipFreeAsync(d_ops, st);
     if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
     return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
@@ -680,7 +681,8 @@ int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n)
     if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
     std::vector<BPass> passes;
     std::vector<BOp> ops;
-    plan_passes(gops, n, n < kBatchMaxTileBits ? n : kBatchMaxTileBits, kSegRegsPass, passes, ops);
+    const int tbt = gate_tuning().blocked_tile_bits;
+    plan_passes(gops, n, n < tbt ? n : tbt, kSegRegsPass, passes, ops);
     return (int)passes.size();
 }
 

@@ -21,6 +21,8 @@
 
 #include <limits.h>
 
+#include <atomic>
+
 #include "batch_device.h"
 #include "sv_device.h"
 #include "sv_internal.h"
@@ -631,22 +633,22 @@ __global__ void k_reset_counter(int *c) { *c = 0; }
 // One cache-blocked pass over a large state: blocks walk the tiles of the
 // pass's tile set (grid-stride), each tile streamed HBM -> LDS, every op of
 // the pass applied in LDS, then streamed back.
-constexpr int kPassThreads = 512;   // two (complex128) or four (complex64) workgroups per CU overlap their phases
 
 // One cache-blocked pass over a large state: each workgroup streams tiles of
 // 2^P.nbits amplitudes through LDS and applies the pass's ops there.  The
 // next tile's loads are issued before the current tile's ops run, and the
 // barriers order LDS only, so loads, LDS work and the previous tile's stores
 // overlap.  NT: streaming accesses (runs are >= 16 amplitudes, whole lines).
-template <typename R, bool NT>
-__global__ __launch_bounds__(kPassThreads) void k_tile_pass(cplx_t<R> *__restrict__ a, BPass P,
-                                                            const BOp *__restrict__ ops, uint64_t ntiles)
+template <typename R, bool NT, int THREADS, int TB>
+__global__ __launch_bounds__(THREADS) void k_tile_pass(cplx_t<R> *__restrict__ a, BPass P,
+                                                       const BOp *__restrict__ ops, uint64_t ntiles)
 {
-    constexpr int PER = (1 << kBatchMaxTileBits) / kPassThreads;
-    __shared__ cplx_t<R> t[1 << kBatchMaxTileBits];
+    constexpr int PER = (1 << TB) / THREADS;
+    static_assert(PER >= 1 && (1 << TB) / 64 <= 64, "tile shape");
+    __shared__ cplx_t<R> t[1 << TB];
     __shared__ uint64_t dlo[64], dhi[64];
     const int tid = threadIdx.x;
-    const int T = 1 << P.nbits;
+    const int T = 1 << P.nbits;   // P.nbits <= TB (checked by the launcher)
     if (tid < 128) {
         const int j = tid & 63, hi = tid >> 6;
         uint64_t o = 0;
@@ -658,55 +660,85 @@ __global__ __launch_bounds__(kPassThreads) void k_tile_pass(cplx_t<R> *__restric
         else dlo[j] = o;
     }
     __syncthreads();
-    // thread tid moves tile amplitudes l = tid + kPassThreads i (l % 64 = tid % 64)
+    // thread tid moves tile amplitudes l = tid + THREADS i (l % 64 = tid % 64)
     const uint64_t lo = dlo[tid & 63];
     const int hi0 = tid >> 6;
-    constexpr int HSTEP = kPassThreads / 64;
+    constexpr int HSTEP = THREADS / 64;
     cplx_t<R> pre[PER];
     uint64_t ti = blockIdx.x;
     if (ti < ntiles) {
         const uint64_t base = tile_base(ti, P) | lo;
 #pragma unroll
         for (int i = 0; i < PER; i++)
-            if (tid + kPassThreads * i < T) pre[i] = ld<NT>(&a[base | dhi[hi0 + HSTEP * i]]);
+            if (tid + THREADS * i < T) pre[i] = ld<NT>(&a[base | dhi[hi0 + HSTEP * i]]);
     }
     for (; ti < ntiles; ti += gridDim.x) {
 #pragma unroll
         for (int i = 0; i < PER; i++)
-            if (tid + kPassThreads * i < T) t[tid + kPassThreads * i] = pre[i];
+            if (tid + THREADS * i < T) t[tid + THREADS * i] = pre[i];
         lds_barrier();
         const uint64_t nti = ti + gridDim.x;
         if (nti < ntiles) {
             const uint64_t nbase = tile_base(nti, P) | lo;
 #pragma unroll
             for (int i = 0; i < PER; i++)
-                if (tid + kPassThreads * i < T) pre[i] = ld<NT>(&a[nbase | dhi[hi0 + HSTEP * i]]);
+                if (tid + THREADS * i < T) pre[i] = ld<NT>(&a[nbase | dhi[hi0 + HSTEP * i]]);
         }
         const uint64_t base = tile_base(ti, P);
-        run_pass_ops<R, kPassThreads, kSegRegsPass, true>(t, T, ops + P.first_op, P.num_ops, base);
+        run_pass_ops<R, THREADS, kSegRegsPass, true>(t, T, ops + P.first_op, P.num_ops, base);
 #pragma unroll
         for (int i = 0; i < PER; i++)
-            if (tid + kPassThreads * i < T) st<NT>(&a[(base | lo) | dhi[hi0 + HSTEP * i]], t[tid + kPassThreads * i]);
+            if (tid + THREADS * i < T) st<NT>(&a[(base | lo) | dhi[hi0 + HSTEP * i]], t[tid + THREADS * i]);
         lds_barrier();   // every thread has read its write-back values before the next tile lands
     }
 }
 
+// Persistent grid of one resident wave of workgroups for this instantiation.
+template <typename R, bool NT, int THREADS, int TB>
+hipError_t run_tile_pass(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, hipStream_t st)
+{
+    static std::atomic<int> per_cu_cached{0};
+    int per_cu = per_cu_cached.load(std::memory_order_relaxed);
+    if (!per_cu) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_pass<R, NT, THREADS, TB>, THREADS, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        per_cu_cached.store(per_cu, std::memory_order_relaxed);
+    }
+    const uint64_t want = (uint64_t)cus * (uint64_t)per_cu;
+    const unsigned grid = (unsigned)(ntiles < want ? ntiles : want);
+    k_tile_pass<R, NT, THREADS, TB><<<grid, THREADS, 0, st>>>((cplx_t<R> *)amps, P, d_ops, ntiles);
+    return hipGetLastError();
+}
+
+template <typename R, bool NT>
+hipError_t tile_pass_shape(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, int threads,
+                           int tb, hipStream_t st)
+{
+    if (tb <= 11) {
+        if (threads <= 256) return run_tile_pass<R, NT, 256, 11>(amps, P, d_ops, ntiles, cus, st);
+        return run_tile_pass<R, NT, 512, 11>(amps, P, d_ops, ntiles, cus, st);
+    }
+    if (threads <= 256) return run_tile_pass<R, NT, 256, 12>(amps, P, d_ops, ntiles, cus, st);
+    return run_tile_pass<R, NT, 512, 12>(amps, P, d_ops, ntiles, cus, st);
+}
+
 }  // namespace
 
-hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int max_blocks,
-                            hipStream_t st)
+// One pass of the single-state blocked path; tile_bits (11 or 12) is the
+// LDS tile the kernel is built for (>= P.nbits), threads its workgroup size.
+hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int threads,
+                            int tile_bits, hipStream_t st)
 {
+    if (P.nbits > tile_bits || tile_bits > kBatchMaxTileBits) return hipErrorInvalidValue;
     const uint64_t ntiles = 1ull << (n - P.nbits);
-    const unsigned grid = (unsigned)(ntiles < (uint64_t)max_blocks ? ntiles : (uint64_t)max_blocks);
     const bool nt = ((1ull << n) * (prec == 1 ? 8ull : 16ull)) >= gate_tuning().nt_min_bytes;
-    if (prec == 1) {
-        if (nt) k_tile_pass<float, true><<<grid, kPassThreads, 0, st>>>((float2 *)amps, P, d_ops, ntiles);
-        else k_tile_pass<float, false><<<grid, kPassThreads, 0, st>>>((float2 *)amps, P, d_ops, ntiles);
-    } else {
-        if (nt) k_tile_pass<double, true><<<grid, kPassThreads, 0, st>>>((double2 *)amps, P, d_ops, ntiles);
-        else k_tile_pass<double, false><<<grid, kPassThreads, 0, st>>>((double2 *)amps, P, d_ops, ntiles);
-    }
-    return hipGetLastError();
+    if (prec == 1)
+        return nt ? tile_pass_shape<float, true>(amps, P, d_ops, ntiles, cus, threads, tile_bits, st)
+                  : tile_pass_shape<float, false>(amps, P, d_ops, ntiles, cus, threads, tile_bits, st);
+    return nt ? tile_pass_shape<double, true>(amps, P, d_ops, ntiles, cus, threads, tile_bits, st)
+              : tile_pass_shape<double, false>(amps, P, d_ops, ntiles, cus, threads, tile_bits, st);
 }
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,

@@ -25,7 +25,7 @@ GateTuning &gate_tuning()
 {
     // One pair per thread and 256-thread blocks measured fastest on MI355X for
     // every gate family at 30 qubits (2, 4, 8 pairs per thread: 2-10 % slower).
-    static GateTuning t = {1, 256, 512ull << 20};
+    static GateTuning t = {1, 256, 512ull << 20, 12, 512};
     return t;
 }
 

@@ -25,6 +25,8 @@ struct GateTuning {
     int ept;     // 1, 2, 4 or 8
     int block;   // 64, 128, 256, 512 or 1024
     uint64_t nt_min_bytes;   // states at least this large use streaming loads/stores
+    int blocked_tile_bits = 12;   // LDS tile of the single-state blocked path: 11 or 12 qubits
+    int blocked_threads = 512;    // its workgroup size: 256 or 512
 };
 GateTuning &gate_tuning();
 

@@ -175,6 +175,20 @@ moonlab_hip_status_t moonlab_hip_set_gate_tuning(int ept, int block)
     return MOONLAB_HIP_OK;
 }
 
+moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads)
+{
+    GateTuning &t = gate_tuning();
+    if (tile_bits) {
+        if (tile_bits != 11 && tile_bits != 12) return fail(MOONLAB_HIP_ERR_BAD_ARG, "tile_bits is 11 or 12");
+        t.blocked_tile_bits = tile_bits;
+    }
+    if (threads) {
+        if (threads != 256 && threads != 512) return fail(MOONLAB_HIP_ERR_BAD_ARG, "threads is 256 or 512");
+        t.blocked_threads = threads;
+    }
+    return MOONLAB_HIP_OK;
+}
+
 moonlab_hip_status_t moonlab_hip_set_streaming_threshold(uint64_t bytes)
 {
     gate_tuning().nt_min_bytes = bytes;
