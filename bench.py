@@ -164,6 +164,49 @@ def run_batch_workload(m, count: int, steps: int, warmup: int, cpu_seconds: floa
     return out
 
 
+def serving_cpu_baseline(reqs, budget_s: float) -> dict | None:
+    """The reference's request path on the host: moonlab_qgtl_execute (build,
+    simulate, 1024 shots) for the requests in arrival order until `budget_s`."""
+    try:
+        from oracle.oracle import RefLib
+        ref = RefLib()
+    except (FileNotFoundError, OSError) as e:
+        print(f"[bench] serving cpu baseline unavailable: {e}", file=sys.stderr, flush=True)
+        return None
+    done = 0
+    t0 = time.perf_counter()
+    for r in reqs:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        ref.qgtl_execute(r["n"], r["gates"], r["shots"], r["shot_seed"])
+        done += 1
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(done / dt, 3), "unit": "circuits/s", "cores": cores, "kind": "reference",
+            "sample": f"first {done} of {len(reqs)} requests in arrival order ({dt:.1f} s), each through "
+                      "tsotchke/moonlab's moonlab_qgtl_execute (complex128, 1024 shots) built by oracle/Makefile"}
+
+
+def complex64_accuracy(m, reqs, done, max_qubits: int = 22) -> dict:
+    """Served complex64 results against FP64: <Z_q> of each served request vs
+    the engine's complex128 run of the same circuit, and the fidelity of the
+    complex64 state with the complex128 state (mixed-precision inner product)."""
+    dz, fid, k = 0.0, 1.0, 0
+    for r, d in zip(reqs, done):
+        if r["precision"] != m.C64 or r["n"] > max_qubits or d.status != 0:
+            continue
+        with m.StateVector(r["n"], m.C128) as s128, m.StateVector(r["n"], m.C64) as s64:
+            s128.apply(r["gates"])
+            z, _ = s128.expect_z_all()
+            s64.apply(r["gates"])
+            ov = s64.inner(s128)
+            fid = min(fid, abs(ov) ** 2 / (s64.norm() * s128.norm()))
+        dz = max(dz, float(np.max(np.abs(np.asarray(d.expect_z) - np.asarray(z)))))
+        k += 1
+    return {"requests_checked": k, "max_qubits": max_qubits, "max_abs_dz_vs_fp64": dz,
+            "min_fidelity_vs_fp64": fid}
+
+
 def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
     """Poisson arrivals submitted in real time; latency = finish - arrival."""
     import threading
@@ -201,6 +244,7 @@ def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
         st = srv.stats()
     finally:
         srv.close()
+    accuracy = None if sequential else complex64_accuracy(m, reqs, done)
     lat = np.array([d.finish_time - d.submit_time for d in done])
     first = min(d.submit_time for d in done)
     last = max(d.finish_time for d in done)
@@ -218,9 +262,48 @@ def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
             "by_path": by_path,
+            "complex64_vs_fp64": accuracy,
             "makespan_s": round(last - first, 3), "failed": bad,
             "batches": int(st["batches"]), "largest_batch": int(st["largest_batch"]),
             "peak_admitted_GiB": round(st["peak_bytes_admitted"] / 2 ** 30, 3)}
+
+
+def fusion_cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | None:
+    """The reference's C (complex128, its only precision) on a prefix of the
+    same n-qubit circuit, gate by gate, until `budget_s` (state init excluded);
+    skipped when the host lacks memory for the state."""
+    try:
+        import psutil
+        from oracle.oracle import RefLib, gates_array
+        ref = RefLib()
+    except (ImportError, FileNotFoundError, OSError) as e:
+        print(f"[bench] fusion cpu baseline unavailable: {e}", file=sys.stderr, flush=True)
+        return None
+    need = (1 << n) * 16
+    if psutil.virtual_memory().available < 2 * need:
+        print(f"[bench] fusion cpu baseline skipped: {need / 2**30:.0f} GiB state does not fit the host",
+              file=sys.stderr, flush=True)
+        return None
+    t_init = time.perf_counter()
+    s = ref.L.ref_state_new(n)
+    if not s:
+        return None
+    t_init = time.perf_counter() - t_init
+    try:
+        ga = gates_array([tuple(g) for g in gates])
+        done = 0
+        t0 = time.perf_counter()
+        while done < len(ga) and time.perf_counter() - t0 < budget_s:
+            if ref.L.ref_apply_gate(s, ga[done:done + 1].ctypes.data):
+                raise RuntimeError(f"reference gate {done} failed")
+            done += 1
+        dt = time.perf_counter() - t0
+    finally:
+        ref.free(s)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(done / dt, 4), "unit": "gates/s", "cores": cores, "kind": "reference",
+            "sample": f"first {done} of {len(gates)} gates of the same {n}-qubit circuit in complex128 "
+                      f"({dt:.1f} s; state init {t_init:.1f} s excluded), tsotchke/moonlab C built by oracle/Makefile"}
 
 
 def run_fusion_workload(m, depth: int, cases) -> dict:
@@ -289,6 +372,7 @@ def run_fusion_workload(m, depth: int, cases) -> dict:
             "norm": nrm,
             "fp64_comparison": "cache-blocked complex128 path",
             "fp64_comparison_seconds": round(t_ref, 3),
+            "gates_per_s": round(len(g) / t_fused, 2),
         }
     return out
 
@@ -331,6 +415,10 @@ def main() -> int:
         t0 = time.perf_counter()
         res = run_fusion_workload(m, args.fusion_depth, cases)
         wall = time.perf_counter() - t0
+        cpu = None
+        if not args.no_cpu:
+            n0 = cases[0][0]
+            cpu = fusion_cpu_baseline(n0, workloads.su2_cz_circuit(n0, args.fusion_depth, SEED + n0), 20.0)
         first = next(iter(res.values()))
         print(json.dumps({
             "metric": "fidelity and <Z> error vs FP64, MFMA TFLOP/s on fused SU(2)+CZ circuits",
@@ -340,7 +428,7 @@ def main() -> int:
             "config": {"workload": f"random SU(2) per qubit + CZ brickwork, depth {args.fusion_depth}, "
                                    "fused into <= 5-qubit blocks (FP64 composition) on MFMA",
                        "cases": args.fusion_cases},
-            "cases": res}), flush=True)
+            "cases": res, "cpu_baseline": cpu}), flush=True)
         return 0
 
     n = args.qubits
@@ -421,11 +509,16 @@ def main() -> int:
     if not args.no_serve:
         served = run_serving_workload(m, args.serve_count, args.serve_rate, False)
         plain = run_serving_workload(m, args.serve_count, args.serve_rate, True)
+        cpu_serve = None
+        if not args.no_cpu:
+            cpu_serve = serving_cpu_baseline(workloads.serving_requests(args.serve_count, SEED + 5, args.serve_rate),
+                                             15.0)
         serving = {"workload": f"{args.serve_count} Poisson requests at {args.serve_rate:g}/s: random-layer "
                                "circuits, n in 10..30 with P(n) ~ 2^(-(n-10)/4), depth ~ U{10..50}, "
                                "complex128/complex64 50/50, 1024 shots + <Z_q> each, 256 GiB HBM budget",
                    "served": served, "plain_sequential": plain,
-                   "circuits_per_s_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2)}
+                   "circuits_per_s_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2),
+                   "cpu_baseline": cpu_serve}
     out = {
         "metric": "HBM TB/s on 30q gate sweep",
         "value": round(value, 4),

@@ -207,6 +207,17 @@ class Oracle:
         return mat
 
 
+class QgtlOptions(ctypes.Structure):   # moonlab_qgtl_options_t (moonlab_qgtl_backend.h:90-95)
+    _fields_ = [("num_shots", ctypes.c_int), ("rng_seed", ctypes.c_uint64),
+                ("return_probabilities", ctypes.c_int)]
+
+
+class QgtlResults(ctypes.Structure):   # moonlab_qgtl_results_t (moonlab_qgtl_backend.h:97-109)
+    _fields_ = [("num_qubits", ctypes.c_int), ("num_shots", ctypes.c_int),
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
+                ("probabilities", ctypes.POINTER(ctypes.c_double))]
+
+
 class RefLib:
     """The reference's own C (oracle/_ref/libmoonlab_ref.so)."""
 
@@ -237,7 +248,37 @@ class RefLib:
         L.measurement_all_qubits.restype = ctypes.c_uint64
         L.quantum_state_clone.argtypes = [vp, vp]
         L.quantum_state_clone.restype = ctypes.c_int
+        ip = ctypes.c_int
+        L.moonlab_qgtl_circuit_create.restype = vp
+        L.moonlab_qgtl_circuit_create.argtypes = [ip]
+        L.moonlab_qgtl_add_gate.argtypes = [vp, ip, ip, ip, dp]
+        L.moonlab_qgtl_execute.argtypes = [vp, ctypes.POINTER(QgtlOptions), ctypes.POINTER(QgtlResults)]
+        L.moonlab_qgtl_results_free.argtypes = [ctypes.POINTER(QgtlResults)]
+        L.moonlab_qgtl_circuit_free.argtypes = [vp]
         self.L = L
+
+    def qgtl_execute(self, n: int, gates, shots: int, seed: int) -> np.ndarray:
+        """One request through the reference's circuit API (moonlab_qgtl_execute:
+        build the circuit, simulate, draw `shots` outcomes); returns the outcomes."""
+        L = self.L
+        c = L.moonlab_qgtl_circuit_create(n)
+        if not c:
+            raise MemoryError("moonlab_qgtl_circuit_create failed")
+        try:
+            for g in gates:
+                t, tgt, ctl, th = int(g[0]), int(g[1]), int(g[2]), float(g[4])
+                p = (ctypes.c_double * 1)(th) if t in (RX, RY, RZ) else None
+                if L.moonlab_qgtl_add_gate(c, t, tgt, ctl, p) != 0:
+                    raise ValueError("reference rejected a gate")
+            o = QgtlOptions(shots, seed, 0)
+            r = QgtlResults()
+            if L.moonlab_qgtl_execute(c, ctypes.byref(o), ctypes.byref(r)) != 0:
+                raise ValueError("moonlab_qgtl_execute failed")
+            out = np.array([r.outcomes[i] for i in range(shots)], dtype=np.uint64)
+            L.moonlab_qgtl_results_free(ctypes.byref(r))
+            return out
+        finally:
+            L.moonlab_qgtl_circuit_free(c)
 
     def run(self, n: int, gates, want_probs: bool = False):
         g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
