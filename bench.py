@@ -164,27 +164,34 @@ def run_batch_workload(m, count: int, steps: int, warmup: int, cpu_seconds: floa
     return out
 
 
-def serving_cpu_baseline(reqs, budget_s: float) -> dict | None:
+def serving_cpu_baseline(reqs, budget_s: float, max_qubits: int = 18) -> dict | None:
     """The reference's request path on the host: moonlab_qgtl_execute (build,
-    simulate, 1024 shots) for the requests in arrival order until `budget_s`."""
+    simulate, 1024 shots) for the requests of at most `max_qubits` qubits in
+    arrival order until `budget_s` (one larger request alone can take the
+    host minutes: the reference clones the state for every shot)."""
     try:
         from oracle.oracle import RefLib
         ref = RefLib()
     except (FileNotFoundError, OSError) as e:
         print(f"[bench] serving cpu baseline unavailable: {e}", file=sys.stderr, flush=True)
         return None
+    small = [r for r in reqs if r["n"] <= max_qubits]
     done = 0
-    t0 = time.perf_counter()
-    for r in reqs:
+    t0 = last = time.perf_counter()
+    for r in small:
         if time.perf_counter() - t0 > budget_s:
             break
         ref.qgtl_execute(r["n"], r["gates"], r["shots"], r["shot_seed"])
         done += 1
+        if time.perf_counter() - last > 5.0:
+            last = time.perf_counter()
+            print(f"[bench] serving cpu baseline: {done} requests", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(done / dt, 3), "unit": "circuits/s", "cores": cores, "kind": "reference",
-            "sample": f"first {done} of {len(reqs)} requests in arrival order ({dt:.1f} s), each through "
-                      "tsotchke/moonlab's moonlab_qgtl_execute (complex128, 1024 shots) built by oracle/Makefile"}
+            "sample": f"first {done} of the {len(small)} requests with n <= {max_qubits} (of {len(reqs)}) in "
+                      f"arrival order ({dt:.1f} s), each through tsotchke/moonlab's moonlab_qgtl_execute "
+                      "(complex128, 1024 shots) built by oracle/Makefile"}
 
 
 def complex64_accuracy(m, reqs, done, max_qubits: int = 22) -> dict:
@@ -226,6 +233,9 @@ def run_serving_workload(m, count: int, rate: float, sequential: bool) -> dict:
             for i in range(len(reqs)):
                 ready.acquire()
                 done[i] = srv.wait(tickets[i])
+                if (i + 1) % 50 == 0:
+                    print(f"[bench] serving ({'plain' if sequential else 'served'}) completed {i + 1}/{len(reqs)}",
+                          file=sys.stderr, flush=True)
 
         th = threading.Thread(target=collect)
         th.start()
@@ -284,6 +294,8 @@ def fusion_cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | No
         print(f"[bench] fusion cpu baseline skipped: {need / 2**30:.0f} GiB state does not fit the host",
               file=sys.stderr, flush=True)
         return None
+    print(f"[bench] fusion cpu baseline: allocating the {need / 2**30:.0f} GiB reference state", file=sys.stderr,
+          flush=True)
     t_init = time.perf_counter()
     s = ref.L.ref_state_new(n)
     if not s:
@@ -293,10 +305,14 @@ def fusion_cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | No
         ga = gates_array([tuple(g) for g in gates])
         done = 0
         t0 = time.perf_counter()
+        last = t0
         while done < len(ga) and time.perf_counter() - t0 < budget_s:
             if ref.L.ref_apply_gate(s, ga[done:done + 1].ctypes.data):
                 raise RuntimeError(f"reference gate {done} failed")
             done += 1
+            if time.perf_counter() - last > 5.0:
+                last = time.perf_counter()
+                print(f"[bench] fusion cpu baseline: {done} gates", file=sys.stderr, flush=True)
         dt = time.perf_counter() - t0
     finally:
         ref.free(s)
