@@ -207,7 +207,8 @@ void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vecto
     while (i < in.size()) {
         uint32_t regs = 0;
         size_t j = i;
-        while (j < in.size() && __builtin_popcount(regs | need(in[j])) <= seg_regs) regs |= need(in[j++]);
+        while (j < in.size() && (int)(j - i) < kOpBuf - 1 && __builtin_popcount(regs | need(in[j])) <= seg_regs)
+            regs |= need(in[j++]);
         if (j - i < 2) {
             out.push_back(in[i++]);
             continue;

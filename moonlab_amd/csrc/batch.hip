@@ -47,6 +47,7 @@ struct Shared {
     double   red[NW][kMaxObs];               // per-wave partials
     double   acc[kMaxObs];                   // per-circuit accumulators
     double   res[8];                         // block_reduce results
+    BOp      opbuf[kOpBuf];                  // op descriptors of the current chunk
     int      next;
     int      flag;
     int      last_nz;
@@ -305,6 +306,38 @@ __device__ __forceinline__ void run_pass_ops(cplx_t<R> *t, int T, const BOp *ops
     }
 }
 
+// run_pass_ops for the batch kernel: a circuit's op descriptors are read
+// once per tile, and a read straight from device memory would expose a full
+// memory latency per op, so they are staged in LDS kOpBuf at a time (one
+// cooperative 16-byte-per-thread copy, one barrier) and applied from there.
+template <typename R, int NR>
+__device__ void run_pass_ops_staged(Shared &sh, cplx_t<R> *t, int T, const BOp *ops, int nops, uint64_t base)
+{
+    constexpr int kWords = (int)(sizeof(BOp) / sizeof(uint4));
+    static_assert(sizeof(BOp) % sizeof(uint4) == 0, "BOp is copied as 16-byte words");
+    for (int k = 0; k < nops;) {
+        const int cnt = nops - k < kOpBuf ? nops - k : kOpBuf;
+        const uint4 *src = (const uint4 *)(ops + k);
+        uint4 *dst = (uint4 *)sh.opbuf;
+        for (int w = threadIdx.x; w < cnt * kWords; w += NT) dst[w] = src[w];
+        __syncthreads();
+        int j = 0;
+        while (j < cnt) {
+            const BOp &op = sh.opbuf[j];
+            if (op.kind == BOP_SEG) {
+                if (j + 1 + op.lq > cnt) break;   // the segment runs past this chunk: restage from it
+                apply_segment<R, NT, NR>(t, T, &sh.opbuf[j + 1], op.lq, op.cl, base);
+                j += 1 + op.lq;
+            } else {
+                apply_op<R, NT>(t, T, op, base);
+                j++;
+            }
+            __syncthreads();
+        }
+        k += j;   // every thread has passed the last barrier: opbuf may be overwritten
+    }
+}
+
 __device__ void build_deposit(Shared &sh, const BPass &p)
 {
     const int tid = threadIdx.x;
@@ -450,7 +483,7 @@ __device__ void run_circuit(Shared &sh, const BInst &I, const BPass *passes, con
                 for (int l = tid; l < T; l += NT) t[l] = slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]];
             }
             __syncthreads();
-            run_pass_ops<R, NT, kSegRegsBatch, false>(t, T, ops + P.first_op, P.num_ops, base);
+            run_pass_ops_staged<R, kSegRegsBatch>(sh, t, T, ops + P.first_op, P.num_ops, base);
             if (!resident) {
                 for (int l = tid; l < T; l += NT) slot[base | sh.dlo[l & 63] | sh.dhi[l >> 6]] = t[l];
                 __syncthreads();
@@ -616,7 +649,10 @@ __global__ __launch_bounds__(NT) void k_batch(const BInst *__restrict__ insts,
     for (;;) {
         if (threadIdx.x == 0) sh.next = atomicAdd(counter, 1);
         __syncthreads();
-        const int k = sh.next;
+        // every lane read the same queue slot: make that explicit, so the
+        // instance, its passes and every op descriptor below are fetched with
+        // scalar (wave-uniform) loads instead of one vector load per lane
+        const int k = __builtin_amdgcn_readfirstlane(sh.next);
         __syncthreads();
         if (k >= ninst) break;
         const BInst I = insts[order[k]];
@@ -629,10 +665,6 @@ __global__ __launch_bounds__(NT) void k_batch(const BInst *__restrict__ insts,
 }
 
 __global__ void k_reset_counter(int *c) { *c = 0; }
-
-// One cache-blocked pass over a large state: blocks walk the tiles of the
-// pass's tile set (grid-stride), each tile streamed HBM -> LDS, every op of
-// the pass applied in LDS, then streamed back.
 
 // One cache-blocked pass over a large state: each workgroup streams tiles of
 // 2^P.nbits amplitudes through LDS and applies the pass's ops there.  The

@@ -21,6 +21,9 @@ enum BOpKind : int32_t {
 
 // Register bits of a segment: the single-state tile pass holds 8 amplitudes
 // per thread; the batch kernel, which also carries measurement state, 4.
+// Op descriptors the batch kernel stages in LDS at a time; a register
+// segment (header + its ops) never exceeds it.
+constexpr int kOpBuf = 32;
 constexpr int kSegRegsPass = 2;
 constexpr int kSegRegsBatch = 2;
 
