@@ -178,10 +178,17 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_sta
  * makes for this gate list on n qubits (host only, no device needed), or a
  * negative moonlab_hip_status_t value. */
 MOONLAB_API int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *gates, int num_gates, int n);
-/* Addition: LDS tile (11 or 12 qubits) and workgroup size (256 or 512) of
- * the blocked path; 0 keeps a value.  Process-wide, like the gate tuning.
- * Results do not depend on it (the same ops in the same order per amplitude). */
+/* Addition: tile (11, 12 or 13 qubits; the register-phase engine uses 12 or
+ * 13, the LDS op engine 11 or 12) and the LDS op engine's workgroup size (256
+ * or 512) of the blocked path; 0 keeps a value.  Process-wide, like the gate
+ * tuning.  Different tiles may reorder commuting gates, so results agree to
+ * rounding; one setting gives the same result on every call. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads);
+/* Addition: engine of the blocked path for states of at least 12 qubits:
+ * 0 = register phases (default: each thread keeps 16 tile amplitudes in
+ * registers, LDS only changes which tile qubits those are), 1 = LDS ops (one
+ * LDS sweep per op or register segment).  Smaller states use the LDS engine. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_set_blocked_engine(int engine);
 
 /* ---- Reductions ---------------------------------------------------------- */
 

@@ -302,6 +302,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_expect_pauli": ([vp, u64, u64, dp], ip),
         "moonlab_hip_blocked_pass_count": ([vp, ip, ip], ip),
         "moonlab_hip_set_blocked_tuning": ([ip, ip], ip),
+        "moonlab_hip_set_blocked_engine": ([ip], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
         "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
@@ -828,9 +829,17 @@ def blocked_pass_count(gates, n: int) -> int:
 
 
 def set_blocked_tuning(tile_bits: int = 0, threads: int = 0) -> None:
-    """LDS tile (11 or 12 qubits) and workgroup size (256 or 512) of the
-    cache-blocked path; 0 keeps a value."""
+    """Tile (11, 12 or 13 qubits) and LDS-engine workgroup size (256 or 512)
+    of the cache-blocked path; 0 keeps a value."""
     _check_hip(lib().moonlab_hip_set_blocked_tuning(tile_bits, threads), "set_blocked_tuning")
+
+
+def set_blocked_engine(engine: str) -> None:
+    """Engine of the cache-blocked path for states of >= 12 qubits:
+    "register" (each thread keeps 16 tile amplitudes in registers; default)
+    or "lds" (one LDS sweep per op or register segment)."""
+    code = {"register": 0, "lds": 1}[engine]
+    _check_hip(lib().moonlab_hip_set_blocked_engine(code), "set_blocked_engine")
 
 
 def set_streaming_threshold(nbytes: int) -> None:

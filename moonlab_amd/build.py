@@ -50,11 +50,18 @@ def sources() -> list[str]:
                   if f.endswith(".hip") or f.endswith(".cpp"))
 
 
+# Per-file device flags.  sv_regpass.hip keeps 16 amplitudes per thread in a
+# register array whose op arms differ only in register indices; sinking the
+# arms' common tails merges them behind address phis that keep the array in
+# scratch memory, so that pass is off there.
+FILE_FLAGS = {"sv_regpass.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON
     if src.endswith(".hip"):
-        cmd += DEVICE + ["-x", "hip"]
+        cmd += DEVICE + FILE_FLAGS.get(os.path.basename(src), []) + ["-x", "hip"]
     cmd += ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -38,6 +38,7 @@ namespace mlx {
 
 hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int threads,
                             int tile_bits, hipStream_t st);
+hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, hipStream_t st);
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
@@ -226,6 +227,142 @@ void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vecto
     }
 }
 
+// A pass's tile-local ops as a register-phase program for k_reg_pass.  Each
+// phase has kRegBits register bits and takes, in order, every op whose
+// non-diagonal target fits its register set and that shares no tile bit with
+// an op deferred from the phase (ops on disjoint qubits commute); the
+// deferred ops, still in order, make up the rest.  Every tile qubit an op
+// touches goes in the register set, so ops never depend on thread bits.  The
+// first phase tries the IO layout (the top kRegBits tile bits), which needs
+// no LDS exchange; a register set is filled up with the highest free bits.
+// SWAP lowers to three exchanges, X_b[a] X_a[b] X_b[a], its controls kept.
+void plan_reg_phases(const std::vector<BOp> &in0, int nbits, std::vector<BOp> &out)
+{
+    std::vector<BOp> in;
+    in.reserve(in0.size());
+    for (const BOp &b : in0) {
+        if (b.kind != BOP_SWAP) {
+            in.push_back(b);
+            continue;
+        }
+        BOp x{};
+        x.kind = BOP_XPERM;
+        x.lq2 = INT32_MIN;
+        x.cg = b.cg;
+        BOp x1 = x, x2 = x;
+        x1.lq = b.lq2;
+        x1.cl = b.cl | (1u << b.lq);
+        x2.lq = b.lq;
+        x2.cl = b.cl | (1u << b.lq2);
+        in.push_back(x1);
+        in.push_back(x2);
+        in.push_back(x1);
+    }
+    // every tile bit an op touches must be a register bit (at most 3: a
+    // target and two controls, or two diagonal qubits)
+    auto need = [](const BOp &b) -> uint32_t {
+        uint32_t m = b.cl;
+        if (b.kind == BOP_MAT1 || b.kind == BOP_XPERM) m |= 1u << b.lq;
+        if (b.kind == BOP_DIAG) {
+            if (b.lq >= 0) m |= 1u << b.lq;
+            if (b.lq2 >= 0) m |= 1u << b.lq2;
+        }
+        return m;
+    };
+    auto touch = need;
+    const uint32_t io = ((1u << kRegBits) - 1) << (nbits - kRegBits);
+    std::vector<const BOp *> rest, later, cur;
+    rest.reserve(in.size());
+    for (const BOp &b : in) rest.push_back(&b);
+    bool first = true;
+    while (!rest.empty()) {
+        uint32_t S = 0;
+        auto scan = [&](bool fixed) {
+            uint32_t blocked = 0;
+            cur.clear();
+            later.clear();
+            for (const BOp *o : rest) {
+                const uint32_t t = touch(*o), nd = need(*o);
+                const bool fits = fixed ? (nd & ~S) == 0 : __builtin_popcount(S | nd) <= kRegBits;
+                if (!(t & blocked) && fits) {
+                    S |= nd;
+                    cur.push_back(o);
+                } else {
+                    blocked |= t;
+                    later.push_back(o);
+                }
+            }
+        };
+        if (first) {
+            S = io;
+            scan(true);
+            if (cur.empty()) {
+                S = 0;
+                scan(false);
+            }
+            first = false;
+        } else {
+            scan(false);
+        }
+        for (int q = nbits - 1; q >= 0 && __builtin_popcount(S) < kRegBits; q--) S |= 1u << q;
+        int r[kRegBits];
+        for (int q = 0, k = 0; q < nbits; q++)
+            if (S & (1u << q)) r[k++] = q;
+        auto roff = [&](int j) {
+            uint32_t o = 0;
+            for (int i = 0; i < kRegBits; i++)
+                if ((j >> i) & 1) o |= 1u << r[i];
+            return o;
+        };
+        auto pos = [&](int q) {
+            for (int i = 0; i < kRegBits; i++)
+                if (r[i] == q) return i;
+            return 15;
+        };
+        BOp h{};
+        h.kind = BOP_PHASE;
+        h.lq = (int32_t)cur.size();
+        h.lq2 = (int32_t)S;
+        for (int i = 0; i < kRegBits; i++) h.cl |= (uint32_t)r[i] << (5 * i);
+        out.push_back(h);
+        for (const BOp *o : cur) {
+            BOp b = *o;
+            const uint32_t creg = o->cl & S;
+            uint32_t jm = 0;
+            for (int j = 0; j < (1 << kRegBits); j++)
+                if ((roff(j) & creg) == creg) jm |= 1u << j;
+            b.cl = 0;   // every local control is a register bit
+            b.pad2 = jm;
+            b.pad = 0;
+            if (o->kind == BOP_DIAG) {
+                // kmap: the local qubits' bits of each register value; with
+                // no global qubit the factor is known per value, and values
+                // whose factor is exactly 1 are left out of jm
+                const int apos = o->lq >= 0 ? pos(o->lq) : -1, bpos = o->lq2 >= 0 ? pos(o->lq2) : -1;
+                const bool global = (o->lq < 0 && o->lq != INT32_MIN) || (o->lq2 < 0 && o->lq2 != INT32_MIN);
+                uint32_t kmap = 0;
+                for (int j = 0; j < (1 << kRegBits); j++) {
+                    const uint32_t k = (apos >= 0 ? (j >> apos) & 1 : 0) | (bpos >= 0 ? ((j >> bpos) & 1) << 1 : 0);
+                    kmap |= k << (2 * j);
+                    if (!global && o->m[2 * k] == 1.0 && o->m[2 * k + 1] == 0.0) b.pad2 &= ~(1u << j);
+                }
+                b.cl = kmap;
+            } else {
+                b.lq = pos(o->lq);   // the scan placed the target in S
+                if (o->kind == BOP_MAT1) {
+                    const double *m = o->m;
+                    if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) b.pad = 1;
+                    else if (m[1] == 0.0 && m[7] == 0.0 && m[2] == 0.0 && m[4] == 0.0) b.pad = 2;
+                } else if (__builtin_popcount(creg) == 1) {   // XPERM with one control
+                    b.pad = 0x10 | pos(__builtin_ctz(creg));
+                }
+            }
+            out.push_back(b);
+        }
+        rest.swap(later);
+    }
+}
+
 // Every qubit an op reads or writes (targets, diagonal qubits, controls).
 uint64_t touches(const GOp &o)
 {
@@ -284,7 +421,8 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
             memcpy(b.m, o->m, sizeof b.m);
             pass_ops.push_back(b);
         }
-        segment_ops(pass_ops, P.nbits, seg_regs, ops);
+        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, ops);
+        else segment_ops(pass_ops, P.nbits, seg_regs, ops);
         P.num_ops = (int)ops.size() - P.first_op;
         passes.push_back(P);
         cur.clear();
@@ -321,6 +459,17 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
 }
 
 namespace {
+
+// Tile bits and engine of the single-state blocked path: the register-phase
+// engine for states of at least 12 qubits (12- or 13-qubit tiles), the LDS
+// op engine otherwise or when selected (11- or 12-qubit tiles).
+int blocked_shape(int n, const GateTuning &t, bool *reg)
+{
+    *reg = t.blocked_engine == 0 && n >= 12;
+    if (*reg) return (t.blocked_tile_bits >= 13 && n >= 13) ? 13 : 12;
+    const int b = t.blocked_tile_bits > 12 ? 12 : t.blocked_tile_bits;
+    return n < b ? n : b;
+}
 
 double now_s()
 {
@@ -649,8 +798,9 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     std::vector<BPass> passes;
     std::vector<BOp> ops;
     const GateTuning tune = gate_tuning();
-    const int tb = n < tune.blocked_tile_bits ? n : tune.blocked_tile_bits;
-    plan_passes(gops, n, tb, kSegRegsPass, passes, ops);
+    bool reg = false;
+    const int tb = blocked_shape(n, tune, &reg);
+    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
     static std::atomic<int> cached_cus{0};   // shared by every launching thread
@@ -667,9 +817,11 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // `ops` is a local vector
     const int prec = moonlab_hip_state_precision(s);
+    void *amps = moonlab_hip_state_device_ptr(s);
+    const int lds_tile = tune.blocked_tile_bits > 12 ? 12 : tune.blocked_tile_bits;
     for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
-        e = launch_tile_pass(moonlab_hip_state_device_ptr(s), prec, n, passes[i], d_ops, cus, tune.blocked_threads,
-                             tune.blocked_tile_bits, st);
+        e = reg ? launch_reg_pass(amps, prec, n, passes[i], d_ops, cus, st)
+                : launch_tile_pass(amps, prec, n, passes[i], d_ops, cus, tune.blocked_threads, lds_tile, st);
     if (d_ops) hipFreeAsync(d_ops, st);
     if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
     return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
@@ -682,8 +834,9 @@ int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n)
     if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
     std::vector<BPass> passes;
     std::vector<BOp> ops;
-    const int tbt = gate_tuning().blocked_tile_bits;
-    plan_passes(gops, n, n < tbt ? n : tbt, kSegRegsPass, passes, ops);
+    bool reg = false;
+    const int tb = blocked_shape(n, gate_tuning(), &reg);
+    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops);
     return (int)passes.size();
 }
 

@@ -2,6 +2,7 @@
 // the host planner (batch.cpp) and the kernel (batch.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace mlx {
@@ -17,7 +18,70 @@ enum BOpKind : int32_t {
     BOP_SEG = 3,    // header: the next lq ops act only on the lq2 (2 or 3) local bits packed
                     // 5 bits each (ascending) in cl, apart from diagonal and control bits;
                     // they run on 2^lq2 amplitudes held in registers per thread
+    BOP_PHASE = 5,  // register-phase program header (k_reg_pass): the next lq ops run on the
+                    // layout whose kRegBits register bits are the tile bits of mask lq2
+                    // (positions packed 5 bits each, ascending, in cl); op fields below
 };
+
+// Register-phase programs (k_reg_pass): each thread holds 2^kRegBits tile
+// amplitudes, value j at tile index tloc | roff(j), roff(j) = sum over i of
+// bit i of j << r_i for the phase's register bits r_0 < .. < r_3.  Ops inside
+// a phase reuse BOp with these meanings:
+// Every tile qubit an op touches (target, controls, diagonal qubits) is a
+// register bit of its phase, so an op acts alike on every thread:
+//   MAT1 / XPERM: lq = register position of the target (0..3); pad2 = 16-bit
+//     mask of the j whose register bits satisfy the local controls (cl = 0);
+//     cg = global controls.  MAT1 pad = the matrix form (0 general, 1 real
+//     entries, 2 real diagonal / imaginary off-diagonal); XPERM pad = 0x10 |
+//     the control's register position when it has exactly one control.
+//   DIAG: lq / lq2 as in the LDS engine (global codes give the factor index
+//     bits kg); cl = kmap, bits 2j..2j+1 = the local qubits' bits of value j
+//     (factor index kg | those bits); pad2 as above, less the values whose
+//     factor is exactly 1 when no qubit is global; cg as above.
+//   SWAP does not occur (it is lowered to three exchanges).
+constexpr int kRegBits = 4;
+
+// Register bits of a phase, ascending tile bits.
+struct RegLayout {
+    int r[kRegBits];
+};
+
+__host__ __device__ inline RegLayout unpack_layout(uint32_t packed)
+{
+    RegLayout L;
+    for (int i = 0; i < kRegBits; i++) L.r[i] = (int)((packed >> (5 * i)) & 31u);
+    return L;
+}
+
+// tile-index bits of register value j
+__host__ __device__ inline int reg_off(const RegLayout &L, int j)
+{
+    int o = 0;
+    for (int i = 0; i < kRegBits; i++)
+        if ((j >> i) & 1) o |= 1 << L.r[i];
+    return o;
+}
+
+// tile index of a thread's register value 0: the thread id spread over the
+// non-register bits
+__host__ __device__ inline int thread_base(const RegLayout &L, int tid)
+{
+    int x = tid;
+    for (int i = 0; i < kRegBits; i++) x = ((x >> L.r[i]) << (L.r[i] + 1)) | (x & ((1 << L.r[i]) - 1));
+    return x;
+}
+
+// LDS slot of tile index x for BYTES-byte amplitudes.  The low bits are
+// XORed with the folded higher bits, so lanes whose indices differ only in
+// higher bits spread over the banks (8 slots of 16 bytes, or 16 of 8 bytes,
+// per 128-byte row).  Linear over GF(2): slot(a | b) = slot(a) ^ slot(b) for
+// disjoint a, b.
+template <int BYTES>
+__host__ __device__ inline int lds_slot(int x)
+{
+    if (BYTES == 16) return x ^ ((x >> 3 ^ x >> 6 ^ x >> 9 ^ x >> 12) & 7);
+    return x ^ ((x >> 4 ^ x >> 8 ^ x >> 12) & 15);
+}
 
 // Register bits of a segment: the single-state tile pass holds 8 amplitudes
 // per thread; the batch kernel, which also carries measurement state, 4.

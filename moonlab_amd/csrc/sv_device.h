@@ -90,6 +90,39 @@ __device__ __forceinline__ void st(float2 *p, const float2 &x)
 }
 
 // Insert a zero bit at position q of p (pair / subspace enumeration).
+// The entries of a diagonal op are read with wave-uniform loads and the one
+// an amplitude needs is selected in registers (an index that differs per lane
+// would turn every read into a vector memory load).
+// readfirstlane moves a value every lane holds into a scalar register; a
+// select between such values stays a select instead of becoming a load from
+// a selected address (which would keep the entries in scratch memory).
+__device__ __forceinline__ float wave_uniform(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// entry k of four values held in registers
+template <typename R>
+__device__ __forceinline__ R pick4(int k, R a, R b, R c, R d)
+{
+    const R lo = (k & 1) ? b : a, hi = (k & 1) ? d : c;
+    return (k & 2) ? hi : lo;
+}
+
+// The four entries of a diagonal op (amplitude *= d[k]), wave-uniform.
+#define MLX_DIAG4(R, op)                                                                                  \
+    const R dr0 = wave_uniform((R)(op).m[0]), di0 = wave_uniform((R)(op).m[1]);                           \
+    const R dr1 = wave_uniform((R)(op).m[2]), di1 = wave_uniform((R)(op).m[3]);                           \
+    const R dr2 = wave_uniform((R)(op).m[4]), di2 = wave_uniform((R)(op).m[5]);                           \
+    const R dr3 = wave_uniform((R)(op).m[6]), di3 = wave_uniform((R)(op).m[7])
+
 __device__ __forceinline__ uint64_t ins0(uint64_t p, int q)
 {
     const uint64_t lo = p & ((1ull << q) - 1ull);

@@ -1,0 +1,329 @@
+// sv_regpass.hip -- register-phase passes of the cache-blocked single-state
+// path (moonlab_hip_apply_gates_blocked, states of 12 qubits or more).
+//
+// A workgroup of 2^(TB - kRegBits) threads streams tiles of 2^TB amplitudes
+// and each thread keeps 2^kRegBits of them in registers.  The ops of a phase
+// act on the register values in place; LDS only carries a change of which
+// tile bits are register bits (one write, one barrier, one read per change).
+// The host planner (batch.cpp: plan_passes, plan_reg_phases) packs a circuit
+// into passes and each pass into phases; the op format is in batch_device.h.
+//
+// Reference counterpart: the per-gate loops of src/quantum/gates.c:123-1048,
+// run here as cache-blocked passes (the reference's CPU engine sweeps the
+// whole state once per gate).
+#include <hip/hip_runtime.h>
+
+#include <limits.h>
+
+#include <atomic>
+
+#include "batch_device.h"
+#include "sv_device.h"
+#include "sv_internal.h"
+
+namespace mlx {
+namespace {
+
+// tile_base for a tile set of exactly TB qubits
+template <int TB>
+__device__ __forceinline__ uint64_t tile_base_c(uint64_t t, const BPass &p)
+{
+#pragma unroll
+    for (int b = 0; b < TB; b++) t = ins0(t, p.qubits[b]);
+    return t;
+}
+
+constexpr int RB = kRegBits, NV = 1 << RB;
+
+// A layout change: every thread writes its register values at their tile
+// indices (LDS slots), then reads the values of the new layout.
+
+template <typename R>
+__device__ __forceinline__ void relayout(cplx_t<R> *t, cplx_t<R> (&v)[NV], int &tloc, const RegLayout &from,
+                                         const RegLayout &to)
+{
+    constexpr int B = (int)sizeof(cplx_t<R>);
+    const int s0 = lds_slot<B>(tloc);
+#pragma unroll
+    for (int j = 0; j < NV; j++) t[s0 ^ lds_slot<B>(reg_off(from, j))] = v[j];
+    lds_barrier();
+    tloc = thread_base(to, (int)threadIdx.x);
+    const int s1 = lds_slot<B>(tloc);
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] = t[s1 ^ lds_slot<B>(reg_off(to, j))];
+}
+
+// 2x2 on register bit P for the register values j whose register-bit
+// controls hold (bit j of jm; MASKED false: all, with no branch between the
+// pairs, so their arithmetic interleaves).  FORM: 0 general (apply_op's
+// BOP_MAT1 arithmetic), 1 real entries (RY, H), 2 real diagonal and
+// imaginary off-diagonal (RX); forms 1 and 2 skip the zero products.
+template <typename R, int P, bool MASKED, int FORM>
+__device__ __forceinline__ void reg_mat1(cplx_t<R> (&v)[NV], const BOp &op, uint32_t jm)
+{
+    const R m0 = (R)op.m[0], m1 = (R)op.m[1], m2 = (R)op.m[2], m3 = (R)op.m[3];
+    const R m4 = (R)op.m[4], m5 = (R)op.m[5], m6 = (R)op.m[6], m7 = (R)op.m[7];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (j & (1 << P)) continue;
+        if (MASKED && !((jm >> j) & 1u)) continue;
+        const cplx_t<R> a = v[j], b = v[j | (1 << P)];
+        cplx_t<R> x, y;
+        if (FORM == 1) {
+            x.x = fmar(m0, a.x, m2 * b.x);
+            x.y = fmar(m0, a.y, m2 * b.y);
+            y.x = fmar(m4, a.x, m6 * b.x);
+            y.y = fmar(m4, a.y, m6 * b.y);
+        } else if (FORM == 2) {   // u00 = m0, u01 = i m3, u10 = i m5, u11 = m6
+            x.x = fmar(m0, a.x, -(m3 * b.y));
+            x.y = fmar(m0, a.y, m3 * b.x);
+            y.x = fmar(m6, b.x, -(m5 * a.y));
+            y.y = fmar(m6, b.y, m5 * a.x);
+        } else {
+            x.x = fmar(m0, a.x, -(m1 * a.y)) + fmar(m2, b.x, -(m3 * b.y));
+            x.y = fmar(m0, a.y, m1 * a.x) + fmar(m2, b.y, m3 * b.x);
+            y.x = fmar(m4, a.x, -(m5 * a.y)) + fmar(m6, b.x, -(m7 * b.y));
+            y.y = fmar(m4, a.y, m5 * a.x) + fmar(m6, b.y, m7 * b.x);
+        }
+        v[j] = x;
+        v[j | (1 << P)] = y;
+    }
+}
+
+template <typename R, bool MASKED, int FORM>
+__device__ __forceinline__ void reg_mat1_at(cplx_t<R> (&v)[NV], const BOp &op, uint32_t jm)
+{
+    switch (op.lq) {
+    case 0: reg_mat1<R, 0, MASKED, FORM>(v, op, jm); break;
+    case 1: reg_mat1<R, 1, MASKED, FORM>(v, op, jm); break;
+    case 2: reg_mat1<R, 2, MASKED, FORM>(v, op, jm); break;
+    default: reg_mat1<R, 3, MASKED, FORM>(v, op, jm); break;
+    }
+}
+
+// controlled X on register bit P for the values selected by jm (any
+// controls)
+template <typename R, int P>
+__device__ __forceinline__ void reg_xperm(cplx_t<R> (&v)[NV], uint32_t jm)
+{
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (j & (1 << P)) continue;
+        if (!((jm >> j) & 1u)) continue;
+        const cplx_t<R> a = v[j];
+        v[j] = v[j | (1 << P)];
+        v[j | (1 << P)] = a;
+    }
+}
+
+// CNOT with target register bit P and control register bit C: a fixed
+// exchange of register values
+template <typename R, int P, int C>
+__device__ __forceinline__ void reg_cx(cplx_t<R> (&v)[NV])
+{
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if ((j & (1 << P)) || !(j & (1 << C))) continue;
+        const cplx_t<R> a = v[j];
+        v[j] = v[j | (1 << P)];
+        v[j | (1 << P)] = a;
+    }
+}
+
+template <typename R, int P>
+__device__ __forceinline__ void reg_cx_from(cplx_t<R> (&v)[NV], int c)
+{
+    // c != P (the host never pairs a target with itself)
+    switch (c) {
+    case 0: if (P != 0) reg_cx<R, P, (P != 0 ? 0 : 1)>(v); break;
+    case 1: if (P != 1) reg_cx<R, P, (P != 1 ? 1 : 0)>(v); break;
+    case 2: if (P != 2) reg_cx<R, P, (P != 2 ? 2 : 0)>(v); break;
+    default: if (P != 3) reg_cx<R, P, (P != 3 ? 3 : 0)>(v); break;
+    }
+}
+
+// diagonal op: value j *= d[k], k = kg | bits 2j..2j+1 of kmap (the local
+// qubits' bits of register value j); kg holds the global qubits' bits
+template <typename R, bool MASKED>
+__device__ __forceinline__ void reg_diag(cplx_t<R> (&v)[NV], const BOp &op, uint32_t jm, uint32_t kmap, int kg)
+{
+    MLX_DIAG4(R, op);
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (MASKED && !((jm >> j) & 1u)) continue;
+        const int k = kg | (int)((kmap >> (2 * j)) & 3u);
+        v[j] = cmul(v[j], pick4(k, dr0, dr1, dr2, dr3), pick4(k, di0, di1, di2, di3));
+    }
+}
+
+// bit of a global qubit code (-1 - q) of the tile's base index; 0 for none
+__device__ __forceinline__ int global_bit(int32_t code, uint64_t base)
+{
+    return (code < 0 && code != INT32_MIN) ? (int)((base >> (-1 - code)) & 1ull) : 0;
+}
+
+// One op of a register phase.  The planner places every tile qubit an op
+// touches (target, controls, diagonal qubits) in the phase's register bits,
+// so the op is the same for every thread of the tile.
+template <typename R>
+__device__ __forceinline__ void reg_op(cplx_t<R> (&v)[NV], const BOp &op, uint64_t base)
+{
+    static_assert(RB == 4, "register positions 0..3");
+    if ((base & op.cg) != op.cg) return;   // a global control is 0 for this whole tile
+    const uint32_t jm = op.pad2;
+    const bool all = jm == (1u << NV) - 1;
+    if (op.kind == BOP_MAT1) {
+        if (!all) reg_mat1_at<R, true, 0>(v, op, jm);
+        else if (op.pad == 1) reg_mat1_at<R, false, 1>(v, op, jm);
+        else if (op.pad == 2) reg_mat1_at<R, false, 2>(v, op, jm);
+        else reg_mat1_at<R, false, 0>(v, op, jm);
+    } else if (op.kind == BOP_XPERM) {
+        if (op.pad & 0x10) {   // one control, a register bit
+            const int c = op.pad & 15;
+            switch (op.lq) {
+            case 0: reg_cx_from<R, 0>(v, c); break;
+            case 1: reg_cx_from<R, 1>(v, c); break;
+            case 2: reg_cx_from<R, 2>(v, c); break;
+            default: reg_cx_from<R, 3>(v, c); break;
+            }
+        } else {
+            switch (op.lq) {
+            case 0: reg_xperm<R, 0>(v, jm); break;
+            case 1: reg_xperm<R, 1>(v, jm); break;
+            case 2: reg_xperm<R, 2>(v, jm); break;
+            default: reg_xperm<R, 3>(v, jm); break;
+            }
+        }
+    } else if (op.kind == BOP_DIAG) {
+        const int kg = __builtin_amdgcn_readfirstlane(global_bit(op.lq, base) | (global_bit(op.lq2, base) << 1));
+        if (all) reg_diag<R, false>(v, op, jm, op.cl, kg);
+        else reg_diag<R, true>(v, op, jm, op.cl, kg);
+    }
+}
+
+// One pass.  A tile enters and leaves in the IO layout, whose register bits
+// are the top kRegBits tile bits: the lanes of a wave then cover the low tile
+// bits, which are runs of contiguous amplitudes.
+template <typename R, bool NT, int TB>
+__global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restrict__ a, BPass P,
+                                                            const BOp *__restrict__ ops, uint64_t ntiles)
+{
+    constexpr int THREADS = 1 << (TB - RB), HSTEP = THREADS / 64, NHI = 1 << (TB - 6);
+    static_assert(THREADS >= 64 && TB <= 13, "tile shape");
+    __shared__ cplx_t<R> t[1 << TB];
+    __shared__ uint64_t dlo[64], dhi[NHI];
+    const int tid = threadIdx.x;
+    // global offsets of tile-index bits 0..5 (dlo) and 6..TB-1 (dhi); the
+    // tile set is indexed with constants only (P.nbits == TB), so it stays in
+    // scalar registers
+    for (int e = tid; e < 64 + NHI; e += THREADS) {
+        uint64_t o = 0;
+        if (e < 64) {
+#pragma unroll
+            for (int b = 0; b < 6; b++)
+                if ((e >> b) & 1) o |= 1ull << P.qubits[b];
+            dlo[e] = o;
+        } else {
+#pragma unroll
+            for (int b = 6; b < TB; b++)
+                if (((e - 64) >> (b - 6)) & 1) o |= 1ull << P.qubits[b];
+            dhi[e - 64] = o;
+        }
+    }
+    __syncthreads();
+    RegLayout io;
+#pragma unroll
+    for (int i = 0; i < RB; i++) io.r[i] = TB - RB + i;
+    const uint32_t io_mask = ((1u << RB) - 1) << (TB - RB);
+    // IO layout: value j of thread tid is tile index tid + THREADS j
+    const uint64_t lo = dlo[tid & 63];
+    const int hi0 = tid >> 6;
+    const BOp *pops = ops + P.first_op;
+    cplx_t<R> v[NV], pre[NV];
+    uint64_t ti = blockIdx.x;
+    if (ti < ntiles) {
+        const uint64_t b0 = tile_base_c<TB>(ti, P) | lo;
+#pragma unroll
+        for (int j = 0; j < NV; j++) pre[j] = ld<NT>(&a[b0 | dhi[hi0 + HSTEP * j]]);
+    }
+    for (; ti < ntiles; ti += gridDim.x) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) v[j] = pre[j];
+        const uint64_t nti = ti + gridDim.x;
+        if (nti < ntiles) {
+            const uint64_t b1 = tile_base_c<TB>(nti, P) | lo;
+#pragma unroll
+            for (int j = 0; j < NV; j++) pre[j] = ld<NT>(&a[b1 | dhi[hi0 + HSTEP * j]]);
+        }
+        const uint64_t base = tile_base_c<TB>(ti, P);
+        int tloc = tid;
+        RegLayout cur = io;
+        uint32_t cur_mask = io_mask;
+        for (int k = 0; k < P.num_ops;) {
+            const BOp &h = pops[k];   // BOP_PHASE header
+            const uint32_t mask = (uint32_t)h.lq2;
+            if (mask != cur_mask) {
+                const RegLayout nl = unpack_layout(h.cl);
+                relayout<R>(t, v, tloc, cur, nl);
+                cur = nl;
+                cur_mask = mask;
+            }
+            // each op's descriptor is fetched while the previous op computes
+            const int cnt = h.lq;
+            BOp op = pops[k + 1];
+            for (int q = 1; q <= cnt; q++) {
+                const BOp nx = pops[k + (q < cnt ? q + 1 : q)];
+                reg_op<R>(v, op, base);
+                op = nx;
+            }
+            k += 1 + cnt;
+        }
+        if (cur_mask != io_mask) relayout<R>(t, v, tloc, cur, io);
+#pragma unroll
+        for (int j = 0; j < NV; j++) st<NT>(&a[(base | lo) | dhi[hi0 + HSTEP * j]], v[j]);
+    }
+}
+
+template <typename R, bool NT, int TB>
+hipError_t run_reg_pass(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, hipStream_t st)
+{
+    constexpr int THREADS = 1 << (TB - RB);
+    static std::atomic<int> per_cu_cached{0};
+    int per_cu = per_cu_cached.load(std::memory_order_relaxed);
+    if (!per_cu) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_reg_pass<R, NT, TB>, THREADS, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        per_cu_cached.store(per_cu, std::memory_order_relaxed);
+    }
+    const uint64_t want = (uint64_t)cus * (uint64_t)per_cu;
+    const unsigned grid = (unsigned)(ntiles < want ? ntiles : want);
+    k_reg_pass<R, NT, TB><<<grid, THREADS, 0, st>>>((cplx_t<R> *)amps, P, d_ops, ntiles);
+    return hipGetLastError();
+}
+
+template <typename R, bool NT>
+hipError_t reg_pass_shape(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, hipStream_t st)
+{
+    if (P.nbits == 13) return run_reg_pass<R, NT, 13>(amps, P, d_ops, ntiles, cus, st);
+    return run_reg_pass<R, NT, 12>(amps, P, d_ops, ntiles, cus, st);
+}
+
+}  // namespace
+
+// One register-phase pass (k_reg_pass) of the single-state blocked path:
+// the pass's tile set has 12 or 13 qubits and n is at least that.
+hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, hipStream_t st)
+{
+    if ((P.nbits != 12 && P.nbits != 13) || n < P.nbits) return hipErrorInvalidValue;
+    const uint64_t ntiles = 1ull << (n - P.nbits);
+    const bool nt = ((1ull << n) * (prec == 1 ? 8ull : 16ull)) >= gate_tuning().nt_min_bytes;
+    if (prec == 1)
+        return nt ? reg_pass_shape<float, true>(amps, P, d_ops, ntiles, cus, st)
+                  : reg_pass_shape<float, false>(amps, P, d_ops, ntiles, cus, st);
+    return nt ? reg_pass_shape<double, true>(amps, P, d_ops, ntiles, cus, st)
+              : reg_pass_shape<double, false>(amps, P, d_ops, ntiles, cus, st);
+}
+
+}  // namespace mlx

@@ -179,13 +179,20 @@ moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads)
 {
     GateTuning &t = gate_tuning();
     if (tile_bits) {
-        if (tile_bits != 11 && tile_bits != 12) return fail(MOONLAB_HIP_ERR_BAD_ARG, "tile_bits is 11 or 12");
+        if (tile_bits < 11 || tile_bits > 13) return fail(MOONLAB_HIP_ERR_BAD_ARG, "tile_bits is 11, 12 or 13");
         t.blocked_tile_bits = tile_bits;
     }
     if (threads) {
         if (threads != 256 && threads != 512) return fail(MOONLAB_HIP_ERR_BAD_ARG, "threads is 256 or 512");
         t.blocked_threads = threads;
     }
+    return MOONLAB_HIP_OK;
+}
+
+moonlab_hip_status_t moonlab_hip_set_blocked_engine(int engine)
+{
+    if (engine != 0 && engine != 1) return fail(MOONLAB_HIP_ERR_BAD_ARG, "engine is 0 or 1");
+    gate_tuning().blocked_engine = engine;
     return MOONLAB_HIP_OK;
 }
 

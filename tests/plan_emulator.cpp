@@ -2,7 +2,9 @@
 // plans of the batch planner (lower + plan_passes, register segments
 // included) on a host state, following the op semantics of the device
 // kernels (apply_op / apply_segment in batch.hip), and compares the result
-// with the oracle's gate-by-gate run.  It checks the host planner -- tile
+// with the oracle's gate-by-gate run.  Register-phase programs (k_reg_pass)
+// run thread by thread with the kernel's LDS slots and layout changes
+// (the layout helpers are the ones batch_device.h gives the kernel).  It checks the host planner -- tile
 // sets, op reordering across passes, segment grouping -- independently of a
 // GPU; the kernels themselves are checked on the GPU against the oracle.
 #include <complex>
@@ -65,6 +67,81 @@ static void apply_segment(std::vector<cd> &t, int T, const mlx::BOp *ops, int no
         for (int j = 0; j < (1 << NR); j++) t[idx[j]] = v[j];
     }
 }
+
+// k_reg_pass on one tile, step by step: 2^(TB-4) threads of 16 register
+// values, LDS slots swizzled for BYTES-byte amplitudes, a layout change =
+// every thread writes, then every thread reads.
+template <int BYTES>
+static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nops, uint64_t base, int &relayouts) {
+    const int RB = mlx::kRegBits, NV = 1 << RB, TH = 1 << (TB - RB);
+    std::vector<cd> lds(1 << TB);
+    std::vector<int> written(1 << TB);
+    std::vector<std::vector<cd>> v(TH, std::vector<cd>(NV));
+    std::vector<int> tloc(TH);
+    mlx::RegLayout io; for (int i = 0; i < RB; i++) io.r[i] = TB - RB + i;
+    const uint32_t io_mask = ((1u << RB) - 1) << (TB - RB);
+    for (int tid = 0; tid < TH; tid++) { tloc[tid] = tid; for (int j = 0; j < NV; j++) v[tid][j] = tile[tid + TH * j]; }
+    auto relayout = [&](const mlx::RegLayout &from, const mlx::RegLayout &to) {
+        relayouts++;
+        std::fill(written.begin(), written.end(), 0);
+        for (int tid = 0; tid < TH; tid++) { int s0 = mlx::lds_slot<BYTES>(tloc[tid]); for (int j = 0; j < NV; j++) { int sl = s0 ^ mlx::lds_slot<BYTES>(mlx::reg_off(from, j)); if (sl < 0 || sl >= (1 << TB)) { printf("slot out of range\n"); exit(1); } written[sl]++; lds[sl] = v[tid][j]; } }
+        for (int x : written) if (x != 1) { printf("layout change does not write every slot once\n"); exit(1); }
+        for (int tid = 0; tid < TH; tid++) { tloc[tid] = mlx::thread_base(to, tid); int s1 = mlx::lds_slot<BYTES>(tloc[tid]); for (int j = 0; j < NV; j++) v[tid][j] = lds[s1 ^ mlx::lds_slot<BYTES>(mlx::reg_off(to, j))]; }
+    };
+    mlx::RegLayout cur = io; uint32_t cur_mask = io_mask;
+    for (int k = 0; k < nops;) {
+        const mlx::BOp &h = ops[k];
+        if (h.kind != mlx::BOP_PHASE || h.lq < 1) { printf("bad phase header\n"); exit(1); }
+        const uint32_t mask = (uint32_t)h.lq2;
+        mlx::RegLayout nl = mlx::unpack_layout(h.cl);
+        uint32_t m2 = 0; for (int i = 0; i < RB; i++) { m2 |= 1u << nl.r[i]; if (i && nl.r[i] <= nl.r[i - 1]) { printf("phase bits not ascending\n"); exit(1); } if (nl.r[i] >= TB) { printf("phase bit outside tile\n"); exit(1); } }
+        if (m2 != mask) { printf("phase mask does not match its bits\n"); exit(1); }
+        if (mask != cur_mask) { relayout(cur, nl); cur = nl; cur_mask = mask; }
+        for (int q = 1; q <= h.lq; q++) {
+            const mlx::BOp &op = ops[k + q];
+            if ((base & op.cg) != op.cg) continue;
+            if (op.kind != mlx::BOP_DIAG && op.cl != 0) { printf("register-phase op with a thread-bit control\n"); exit(1); }
+            const uint32_t jm = op.pad2;
+            const bool all = jm == 0xFFFFu;
+            for (int tid = 0; tid < TH; tid++) {
+                std::vector<cd> &w = v[tid];
+                if (op.kind == mlx::BOP_MAT1) {
+                    const int P = op.lq; if (P < 0 || P >= RB) { printf("register position out of range\n"); exit(1); }
+                    const int form = all ? op.pad : 0;
+                    for (int j = 0; j < NV; j++) {
+                        if (j & (1 << P)) continue;
+                        if (((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
+                        if (!((jm >> j) & 1)) continue;
+                        cd &a = w[j], &b = w[j | (1 << P)];
+                        if (form == 1) { double m0 = op.m[0], m2 = op.m[2], m4 = op.m[4], m6 = op.m[6]; cd x = m0 * a + m2 * b, y = m4 * a + m6 * b; a = x; b = y; }
+                        else if (form == 2) { cd u01(0, op.m[3]), u10(0, op.m[5]); cd x = op.m[0] * a + u01 * b, y = u10 * a + op.m[6] * b; a = x; b = y; }
+                        else mat1(a, b, op);
+                    }
+                } else if (op.kind == mlx::BOP_XPERM) {
+                    const int P = op.lq; if (P < 0 || P >= RB) { printf("register position out of range\n"); exit(1); }
+                    uint32_t sel = jm;
+                    if (op.pad & 0x10) { int c = op.pad & 15; if (c == P || c >= RB) { printf("bad CNOT control\n"); exit(1); } sel = 0; for (int j = 0; j < NV; j++) if (j & (1 << c)) sel |= 1u << j; if (sel != jm) { printf("CNOT control differs from its mask\n"); exit(1); } }
+                    for (int j = 0; j < NV; j++) {
+                        if (j & (1 << P)) continue;
+                        if (((sel >> j) & 1) != ((sel >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
+                        if ((sel >> j) & 1) std::swap(w[j], w[j | (1 << P)]);
+                    }
+                } else if (op.kind == mlx::BOP_DIAG) {
+                    auto gbit = [&](int32_t c) { return (c < 0 && c != INT32_MIN) ? (int)((base >> (-1 - c)) & 1) : 0; };
+                    const int kg = gbit(op.lq) | (gbit(op.lq2) << 1);
+                    for (int j = 0; j < NV; j++) {
+                        if (!((jm >> j) & 1)) continue;
+                        const int kk = kg | (int)((op.cl >> (2 * j)) & 3u);
+                        w[j] *= cd(op.m[2 * kk], op.m[2 * kk + 1]);
+                    }
+                } else { printf("op kind %d in a register phase\n", op.kind); exit(1); }
+            }
+        }
+        k += 1 + h.lq;
+    }
+    if (cur_mask != io_mask) relayout(cur, io);
+    for (int tid = 0; tid < TH; tid++) for (int j = 0; j < NV; j++) tile[tid + TH * j] = v[tid][j];
+}
 int main() {
     std::mt19937_64 rng(9);
     int segs_seen = 0, checked = 0;
@@ -104,5 +181,42 @@ int main() {
             checked++;
         }
     }
-    printf("ok: %d plans checked, %d segment executions\n", checked, segs_seen);
+    // register-phase programs: 12..15 qubits, tiles of 12 and 13, both
+    // amplitude widths of the LDS swizzle
+    int reg_checked = 0, relayouts = 0;
+    for (int trial = 0; trial < 16; trial++) {
+        int n = 12 + trial % 4;
+        std::vector<moonlab_hip_gate_t> g;
+        for (int k = 0; k < 120; k++) {
+            int t = 1 + rng() % 24; int q[3]; for (int i = 0; i < 3; i++) { do { q[i] = rng() % n; } while ((i > 0 && q[i] == q[0]) || (i > 1 && q[i] == q[1])); }
+            g.push_back({t, q[0], q[1], q[2], (double)(rng() % 1000) / 77.0, (double)(rng() % 1000) / 91.0, (double)(rng() % 1000) / 53.0});
+        }
+        std::vector<double> want(2ull << n, 0.0); want[0] = 1.0;
+        svo_apply_gates(n, want.data(), (const svo_gate_t *)g.data(), (int)g.size());
+        for (int tb : {12, 13}) {
+            if (tb > n) continue;
+            std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
+            std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
+            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops);
+            std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
+            for (const mlx::BPass &P : passes) {
+                if (P.nbits != tb) { printf("register pass with %d tile bits\n", P.nbits); return 1; }
+                int T = 1 << P.nbits; uint64_t ntiles = 1ull << (n - P.nbits);
+                for (uint64_t ti = 0; ti < ntiles; ti++) {
+                    uint64_t base = ti; for (int b = 0; b < P.nbits; b++) base = ins0(base, P.qubits[b]);
+                    std::vector<cd> t(T);
+                    auto addr = [&](int l) { uint64_t a = base; for (int b = 0; b < P.nbits; b++) if (l & (1 << b)) a |= 1ull << P.qubits[b]; return a; };
+                    for (int l = 0; l < T; l++) t[l] = st[addr(l)];
+                    if (trial & 1) reg_tile<8>(t, P.nbits, &ops[P.first_op], P.num_ops, base, relayouts);
+                    else reg_tile<16>(t, P.nbits, &ops[P.first_op], P.num_ops, base, relayouts);
+                    for (int l = 0; l < T; l++) st[addr(l)] = t[l];
+                }
+            }
+            double err = 0; for (uint64_t i = 0; i < (1ull << n); i++) err = std::max(err, std::abs(st[i] - cd(want[2 * i], want[2 * i + 1])));
+            if (err > 1e-12) { printf("REGISTER PHASE MISMATCH n=%d tb=%d err=%g\n", n, tb, err); return 1; }
+            reg_checked++;
+        }
+    }
+    printf("ok: %d plans checked, %d segment executions, %d register-phase plans, %d layout changes\n", checked,
+           segs_seen, reg_checked, relayouts);
 }

@@ -473,6 +473,30 @@ def test_cache_blocked_application_vs_oracle(eng, oracle):
     assert r.outcomes.tolist() == oracle.job_shots(n, want, 300, 1, 77).tolist()
 
 
+@pytest.mark.parametrize("engine,tile_bits", [("register", 12), ("register", 13), ("lds", 11), ("lds", 12)])
+def test_blocked_engines_vs_oracle(eng, oracle, engine, tile_bits):
+    """Each blocked engine and tile (register phases on 12- and 13-qubit
+    tiles, LDS ops on 11 and 12) over the whole gate table: complex128 within
+    1e-12 of the oracle, complex64 by fidelity; 24 qubits gives every
+    workgroup several tiles."""
+    rng = np.random.default_rng(4100 + tile_bits)
+    try:
+        eng.set_blocked_engine(engine)
+        eng.set_blocked_tuning(tile_bits, 0)
+        for n, ng in ((12, 200), (13, 200), (14, 200), (17, 160), (24, 60)):
+            gl = random_gate_list(n, ng, rng)
+            want = oracle.run(n, gl)
+            with eng.StateVector(n) as s:
+                got = s.apply(gl, blocked=True).amplitudes()
+            assert np.max(np.abs(got - want)) < TOL, (engine, tile_bits, n)
+            with eng.StateVector(n, eng.C64) as s:
+                got = s.apply(gl, blocked=True).amplitudes()
+            assert abs(np.vdot(want, got)) ** 2 > 1 - 1e-6, (engine, tile_bits, n)
+    finally:
+        eng.set_blocked_engine("register")
+        eng.set_blocked_tuning(12, 512)
+
+
 @pytest.mark.gpu
 def test_plugin_runs_in_reference_scheduler(eng):
     """Registered into the reference's own scheduler (compiled reference C),
