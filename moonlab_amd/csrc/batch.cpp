@@ -334,6 +334,8 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, std::vector<BOp> &o
             b.cl = 0;   // every local control is a register bit
             b.pad2 = jm;
             b.pad = 0;
+            const bool all = jm == (1u << (1 << kRegBits)) - 1;
+            auto one = [](const double *d) { return d[0] == 1.0 && d[1] == 0.0; };
             if (o->kind == BOP_DIAG) {
                 // kmap: the local qubits' bits of each register value; with
                 // no global qubit the factor is known per value, and values
@@ -344,17 +346,28 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, std::vector<BOp> &o
                 for (int j = 0; j < (1 << kRegBits); j++) {
                     const uint32_t k = (apos >= 0 ? (j >> apos) & 1 : 0) | (bpos >= 0 ? ((j >> bpos) & 1) << 1 : 0);
                     kmap |= k << (2 * j);
-                    if (!global && o->m[2 * k] == 1.0 && o->m[2 * k + 1] == 0.0) b.pad2 &= ~(1u << j);
+                    if (!global && one(o->m + 2 * k)) b.pad2 &= ~(1u << j);
                 }
                 b.cl = kmap;
+                b.pad = ARM_DIAG;
+                if (!global && all && apos >= 0 && bpos < 0) {
+                    b.pad = (one(o->m) ? ARM_DIAG1_ONLY1 : ARM_DIAG1) + apos;
+                } else if (!global && all && apos >= 0 && bpos >= 0 && one(o->m) && one(o->m + 2) &&
+                           one(o->m + 4)) {
+                    b.pad = ARM_DIAG11 + 4 * std::min(apos, bpos) + std::max(apos, bpos);
+                }
             } else {
                 b.lq = pos(o->lq);   // the scan placed the target in S
                 if (o->kind == BOP_MAT1) {
                     const double *m = o->m;
-                    if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) b.pad = 1;
-                    else if (m[1] == 0.0 && m[7] == 0.0 && m[2] == 0.0 && m[4] == 0.0) b.pad = 2;
+                    if (!all) b.pad = ARM_MAT1_MASKED + b.lq;
+                    else if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) b.pad = ARM_MAT1_REAL + b.lq;
+                    else if (m[1] == 0.0 && m[7] == 0.0 && m[2] == 0.0 && m[4] == 0.0) b.pad = ARM_MAT1_RX + b.lq;
+                    else b.pad = ARM_MAT1 + b.lq;
                 } else if (__builtin_popcount(creg) == 1) {   // XPERM with one control
-                    b.pad = 0x10 | pos(__builtin_ctz(creg));
+                    b.pad = ARM_CX + 4 * b.lq + pos(__builtin_ctz(creg));
+                } else {
+                    b.pad = ARM_XPERM + b.lq;
                 }
             }
             out.push_back(b);

@@ -29,17 +29,31 @@ enum BOpKind : int32_t {
 // a phase reuse BOp with these meanings:
 // Every tile qubit an op touches (target, controls, diagonal qubits) is a
 // register bit of its phase, so an op acts alike on every thread:
+//   pad = the kernel arm (RegArm below) the host picked for the op's kind,
+//     form and register positions; cg = global controls (the op is skipped
+//     on tiles where one is 0).
 //   MAT1 / XPERM: lq = register position of the target (0..3); pad2 = 16-bit
-//     mask of the j whose register bits satisfy the local controls (cl = 0);
-//     cg = global controls.  MAT1 pad = the matrix form (0 general, 1 real
-//     entries, 2 real diagonal / imaginary off-diagonal); XPERM pad = 0x10 |
-//     the control's register position when it has exactly one control.
+//     mask of the j whose register bits satisfy the local controls (cl = 0).
 //   DIAG: lq / lq2 as in the LDS engine (global codes give the factor index
 //     bits kg); cl = kmap, bits 2j..2j+1 = the local qubits' bits of value j
 //     (factor index kg | those bits); pad2 as above, less the values whose
-//     factor is exactly 1 when no qubit is global; cg as above.
+//     factor is exactly 1 when no qubit is global.
 //   SWAP does not occur (it is lowered to three exchanges).
 constexpr int kRegBits = 4;
+
+// Arms of the register-phase interpreter (P, Q, C = register positions).
+enum RegArm : int32_t {
+    ARM_MAT1_MASKED = 0,    // + P: 2x2 on the values jm selects
+    ARM_MAT1 = 4,           // + P: general 2x2 on every value
+    ARM_MAT1_REAL = 8,      // + P: real 2x2 (RY, H)
+    ARM_MAT1_RX = 12,       // + P: real diagonal, imaginary off-diagonal (RX)
+    ARM_XPERM = 16,         // + P: exchange across P on the values jm selects
+    ARM_CX = 20,            // + 4 P + C: CNOT, target P, control C
+    ARM_DIAG1 = 36,         // + P: diag(d0, d1) on P, no controls
+    ARM_DIAG1_ONLY1 = 40,   // + P: diag(1, d1) on P, no controls
+    ARM_DIAG11 = 44,        // + 4 P + Q (P < Q): diag(1, 1, 1, d3), no controls
+    ARM_DIAG = 60,          // any diagonal (kmap, kg, jm)
+};
 
 // Register bits of a phase, ascending tile bits.
 struct RegLayout {

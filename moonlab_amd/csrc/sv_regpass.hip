@@ -90,17 +90,6 @@ __device__ __forceinline__ void reg_mat1(cplx_t<R> (&v)[NV], const BOp &op, uint
     }
 }
 
-template <typename R, bool MASKED, int FORM>
-__device__ __forceinline__ void reg_mat1_at(cplx_t<R> (&v)[NV], const BOp &op, uint32_t jm)
-{
-    switch (op.lq) {
-    case 0: reg_mat1<R, 0, MASKED, FORM>(v, op, jm); break;
-    case 1: reg_mat1<R, 1, MASKED, FORM>(v, op, jm); break;
-    case 2: reg_mat1<R, 2, MASKED, FORM>(v, op, jm); break;
-    default: reg_mat1<R, 3, MASKED, FORM>(v, op, jm); break;
-    }
-}
-
 // controlled X on register bit P for the values selected by jm (any
 // controls)
 template <typename R, int P>
@@ -130,27 +119,38 @@ __device__ __forceinline__ void reg_cx(cplx_t<R> (&v)[NV])
     }
 }
 
-template <typename R, int P>
-__device__ __forceinline__ void reg_cx_from(cplx_t<R> (&v)[NV], int c)
+// single-qubit diagonal on register bit P: value j *= d[bit P of j]
+// (ONLY1: d0 is exactly 1 and those values are left alone)
+template <typename R, int P, bool ONLY1>
+__device__ __forceinline__ void reg_diag1(cplx_t<R> (&v)[NV], const BOp &op)
 {
-    // c != P (the host never pairs a target with itself)
-    switch (c) {
-    case 0: if (P != 0) reg_cx<R, P, (P != 0 ? 0 : 1)>(v); break;
-    case 1: if (P != 1) reg_cx<R, P, (P != 1 ? 1 : 0)>(v); break;
-    case 2: if (P != 2) reg_cx<R, P, (P != 2 ? 2 : 0)>(v); break;
-    default: if (P != 3) reg_cx<R, P, (P != 3 ? 3 : 0)>(v); break;
+    const R d0r = (R)op.m[0], d0i = (R)op.m[1], d1r = (R)op.m[2], d1i = (R)op.m[3];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (j & (1 << P)) v[j] = cmul(v[j], d1r, d1i);
+        else if (!ONLY1) v[j] = cmul(v[j], d0r, d0i);
     }
 }
 
-// diagonal op: value j *= d[k], k = kg | bits 2j..2j+1 of kmap (the local
-// qubits' bits of register value j); kg holds the global qubits' bits
-template <typename R, bool MASKED>
+// two-qubit diagonal diag(1, 1, 1, d3) on register bits P < Q (CZ, CPHASE)
+template <typename R, int P, int Q>
+__device__ __forceinline__ void reg_diag11(cplx_t<R> (&v)[NV], const BOp &op)
+{
+    const R d3r = (R)op.m[6], d3i = (R)op.m[7];
+#pragma unroll
+    for (int j = 0; j < NV; j++)
+        if ((j & (1 << P)) && (j & (1 << Q))) v[j] = cmul(v[j], d3r, d3i);
+}
+
+// any diagonal op: value j *= d[k], k = kg | bits 2j..2j+1 of kmap (the
+// local qubits' bits of register value j); kg holds the global qubits' bits
+template <typename R>
 __device__ __forceinline__ void reg_diag(cplx_t<R> (&v)[NV], const BOp &op, uint32_t jm, uint32_t kmap, int kg)
 {
     MLX_DIAG4(R, op);
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-        if (MASKED && !((jm >> j) & 1u)) continue;
+        if (!((jm >> j) & 1u)) continue;
         const int k = kg | (int)((kmap >> (2 * j)) & 3u);
         v[j] = cmul(v[j], pick4(k, dr0, dr1, dr2, dr3), pick4(k, di0, di1, di2, di3));
     }
@@ -162,42 +162,68 @@ __device__ __forceinline__ int global_bit(int32_t code, uint64_t base)
     return (code < 0 && code != INT32_MIN) ? (int)((base >> (-1 - code)) & 1ull) : 0;
 }
 
-// One op of a register phase.  The planner places every tile qubit an op
-// touches (target, controls, diagonal qubits) in the phase's register bits,
-// so the op is the same for every thread of the tile.
+// One op of a register phase, dispatched on its arm (BOp::pad, chosen by the
+// host; the numbering is RegArm in batch_device.h).  The planner places every
+// tile qubit an op touches in the phase's register bits, so the op is the
+// same for every thread of the tile.
 template <typename R>
 __device__ __forceinline__ void reg_op(cplx_t<R> (&v)[NV], const BOp &op, uint64_t base)
 {
     static_assert(RB == 4, "register positions 0..3");
     if ((base & op.cg) != op.cg) return;   // a global control is 0 for this whole tile
     const uint32_t jm = op.pad2;
-    const bool all = jm == (1u << NV) - 1;
-    if (op.kind == BOP_MAT1) {
-        if (!all) reg_mat1_at<R, true, 0>(v, op, jm);
-        else if (op.pad == 1) reg_mat1_at<R, false, 1>(v, op, jm);
-        else if (op.pad == 2) reg_mat1_at<R, false, 2>(v, op, jm);
-        else reg_mat1_at<R, false, 0>(v, op, jm);
-    } else if (op.kind == BOP_XPERM) {
-        if (op.pad & 0x10) {   // one control, a register bit
-            const int c = op.pad & 15;
-            switch (op.lq) {
-            case 0: reg_cx_from<R, 0>(v, c); break;
-            case 1: reg_cx_from<R, 1>(v, c); break;
-            case 2: reg_cx_from<R, 2>(v, c); break;
-            default: reg_cx_from<R, 3>(v, c); break;
-            }
-        } else {
-            switch (op.lq) {
-            case 0: reg_xperm<R, 0>(v, jm); break;
-            case 1: reg_xperm<R, 1>(v, jm); break;
-            case 2: reg_xperm<R, 2>(v, jm); break;
-            default: reg_xperm<R, 3>(v, jm); break;
-            }
-        }
-    } else if (op.kind == BOP_DIAG) {
-        const int kg = __builtin_amdgcn_readfirstlane(global_bit(op.lq, base) | (global_bit(op.lq2, base) << 1));
-        if (all) reg_diag<R, false>(v, op, jm, op.cl, kg);
-        else reg_diag<R, true>(v, op, jm, op.cl, kg);
+    switch (op.pad) {
+    case ARM_MAT1_MASKED + 0: reg_mat1<R, 0, true, 0>(v, op, jm); break;
+    case ARM_MAT1_MASKED + 1: reg_mat1<R, 1, true, 0>(v, op, jm); break;
+    case ARM_MAT1_MASKED + 2: reg_mat1<R, 2, true, 0>(v, op, jm); break;
+    case ARM_MAT1_MASKED + 3: reg_mat1<R, 3, true, 0>(v, op, jm); break;
+    case ARM_MAT1 + 0: reg_mat1<R, 0, false, 0>(v, op, jm); break;
+    case ARM_MAT1 + 1: reg_mat1<R, 1, false, 0>(v, op, jm); break;
+    case ARM_MAT1 + 2: reg_mat1<R, 2, false, 0>(v, op, jm); break;
+    case ARM_MAT1 + 3: reg_mat1<R, 3, false, 0>(v, op, jm); break;
+    case ARM_MAT1_REAL + 0: reg_mat1<R, 0, false, 1>(v, op, jm); break;
+    case ARM_MAT1_REAL + 1: reg_mat1<R, 1, false, 1>(v, op, jm); break;
+    case ARM_MAT1_REAL + 2: reg_mat1<R, 2, false, 1>(v, op, jm); break;
+    case ARM_MAT1_REAL + 3: reg_mat1<R, 3, false, 1>(v, op, jm); break;
+    case ARM_MAT1_RX + 0: reg_mat1<R, 0, false, 2>(v, op, jm); break;
+    case ARM_MAT1_RX + 1: reg_mat1<R, 1, false, 2>(v, op, jm); break;
+    case ARM_MAT1_RX + 2: reg_mat1<R, 2, false, 2>(v, op, jm); break;
+    case ARM_MAT1_RX + 3: reg_mat1<R, 3, false, 2>(v, op, jm); break;
+    case ARM_XPERM + 0: reg_xperm<R, 0>(v, jm); break;
+    case ARM_XPERM + 1: reg_xperm<R, 1>(v, jm); break;
+    case ARM_XPERM + 2: reg_xperm<R, 2>(v, jm); break;
+    case ARM_XPERM + 3: reg_xperm<R, 3>(v, jm); break;
+    case ARM_CX + 4 * 0 + 1: reg_cx<R, 0, 1>(v); break;
+    case ARM_CX + 4 * 0 + 2: reg_cx<R, 0, 2>(v); break;
+    case ARM_CX + 4 * 0 + 3: reg_cx<R, 0, 3>(v); break;
+    case ARM_CX + 4 * 1 + 0: reg_cx<R, 1, 0>(v); break;
+    case ARM_CX + 4 * 1 + 2: reg_cx<R, 1, 2>(v); break;
+    case ARM_CX + 4 * 1 + 3: reg_cx<R, 1, 3>(v); break;
+    case ARM_CX + 4 * 2 + 0: reg_cx<R, 2, 0>(v); break;
+    case ARM_CX + 4 * 2 + 1: reg_cx<R, 2, 1>(v); break;
+    case ARM_CX + 4 * 2 + 3: reg_cx<R, 2, 3>(v); break;
+    case ARM_CX + 4 * 3 + 0: reg_cx<R, 3, 0>(v); break;
+    case ARM_CX + 4 * 3 + 1: reg_cx<R, 3, 1>(v); break;
+    case ARM_CX + 4 * 3 + 2: reg_cx<R, 3, 2>(v); break;
+    case ARM_DIAG1 + 0: reg_diag1<R, 0, false>(v, op); break;
+    case ARM_DIAG1 + 1: reg_diag1<R, 1, false>(v, op); break;
+    case ARM_DIAG1 + 2: reg_diag1<R, 2, false>(v, op); break;
+    case ARM_DIAG1 + 3: reg_diag1<R, 3, false>(v, op); break;
+    case ARM_DIAG1_ONLY1 + 0: reg_diag1<R, 0, true>(v, op); break;
+    case ARM_DIAG1_ONLY1 + 1: reg_diag1<R, 1, true>(v, op); break;
+    case ARM_DIAG1_ONLY1 + 2: reg_diag1<R, 2, true>(v, op); break;
+    case ARM_DIAG1_ONLY1 + 3: reg_diag1<R, 3, true>(v, op); break;
+    case ARM_DIAG11 + 4 * 0 + 1: reg_diag11<R, 0, 1>(v, op); break;
+    case ARM_DIAG11 + 4 * 0 + 2: reg_diag11<R, 0, 2>(v, op); break;
+    case ARM_DIAG11 + 4 * 0 + 3: reg_diag11<R, 0, 3>(v, op); break;
+    case ARM_DIAG11 + 4 * 1 + 2: reg_diag11<R, 1, 2>(v, op); break;
+    case ARM_DIAG11 + 4 * 1 + 3: reg_diag11<R, 1, 3>(v, op); break;
+    case ARM_DIAG11 + 4 * 2 + 3: reg_diag11<R, 2, 3>(v, op); break;
+    default: {   // ARM_DIAG
+        const int kg = global_bit(op.lq, base) | (global_bit(op.lq2, base) << 1);
+        reg_diag<R>(v, op, jm, op.cl, __builtin_amdgcn_readfirstlane(kg));
+        break;
+    }
     }
 }
 
@@ -268,14 +294,8 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restri
                 cur = nl;
                 cur_mask = mask;
             }
-            // each op's descriptor is fetched while the previous op computes
             const int cnt = h.lq;
-            BOp op = pops[k + 1];
-            for (int q = 1; q <= cnt; q++) {
-                const BOp nx = pops[k + (q < cnt ? q + 1 : q)];
-                reg_op<R>(v, op, base);
-                op = nx;
-            }
+            for (int q = 1; q <= cnt; q++) reg_op<R>(v, pops[k + q], base);
             k += 1 + cnt;
         }
         if (cur_mask != io_mask) relayout<R>(t, v, tloc, cur, io);

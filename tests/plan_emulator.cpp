@@ -102,31 +102,48 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
             if ((base & op.cg) != op.cg) continue;
             if (op.kind != mlx::BOP_DIAG && op.cl != 0) { printf("register-phase op with a thread-bit control\n"); exit(1); }
             const uint32_t jm = op.pad2;
-            const bool all = jm == 0xFFFFu;
+            const int arm = op.pad;
+            // the kernel's arms (sv_regpass.hip: reg_op), value by value
             for (int tid = 0; tid < TH; tid++) {
                 std::vector<cd> &w = v[tid];
-                if (op.kind == mlx::BOP_MAT1) {
-                    const int P = op.lq; if (P < 0 || P >= RB) { printf("register position out of range\n"); exit(1); }
-                    const int form = all ? op.pad : 0;
+                if (arm < mlx::ARM_XPERM) {   // MAT1 arms
+                    const int P = arm % 4, form = arm < mlx::ARM_MAT1 ? 0 : (arm - mlx::ARM_MAT1) / 4;
+                    const bool masked = arm < mlx::ARM_MAT1;
+                    if (op.kind != mlx::BOP_MAT1 || P != op.lq) { printf("MAT1 arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) {
                         if (j & (1 << P)) continue;
-                        if (((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
-                        if (!((jm >> j) & 1)) continue;
+                        if (masked && ((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
+                        if (masked && !((jm >> j) & 1)) continue;
                         cd &a = w[j], &b = w[j | (1 << P)];
                         if (form == 1) { double m0 = op.m[0], m2 = op.m[2], m4 = op.m[4], m6 = op.m[6]; cd x = m0 * a + m2 * b, y = m4 * a + m6 * b; a = x; b = y; }
                         else if (form == 2) { cd u01(0, op.m[3]), u10(0, op.m[5]); cd x = op.m[0] * a + u01 * b, y = u10 * a + op.m[6] * b; a = x; b = y; }
                         else mat1(a, b, op);
                     }
-                } else if (op.kind == mlx::BOP_XPERM) {
-                    const int P = op.lq; if (P < 0 || P >= RB) { printf("register position out of range\n"); exit(1); }
-                    uint32_t sel = jm;
-                    if (op.pad & 0x10) { int c = op.pad & 15; if (c == P || c >= RB) { printf("bad CNOT control\n"); exit(1); } sel = 0; for (int j = 0; j < NV; j++) if (j & (1 << c)) sel |= 1u << j; if (sel != jm) { printf("CNOT control differs from its mask\n"); exit(1); } }
+                } else if (arm < mlx::ARM_CX) {   // XPERM on jm
+                    const int P = arm - mlx::ARM_XPERM;
+                    if (op.kind != mlx::BOP_XPERM || P != op.lq) { printf("XPERM arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) {
                         if (j & (1 << P)) continue;
-                        if (((sel >> j) & 1) != ((sel >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
-                        if ((sel >> j) & 1) std::swap(w[j], w[j | (1 << P)]);
+                        if (((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
+                        if ((jm >> j) & 1) std::swap(w[j], w[j | (1 << P)]);
                     }
-                } else if (op.kind == mlx::BOP_DIAG) {
+                } else if (arm < mlx::ARM_DIAG1) {   // CNOT, target P, control C
+                    const int P = (arm - mlx::ARM_CX) / 4, C = (arm - mlx::ARM_CX) % 4;
+                    if (op.kind != mlx::BOP_XPERM || P != op.lq || P == C) { printf("CNOT arm mismatch\n"); exit(1); }
+                    for (int j = 0; j < NV; j++) if (!(j & (1 << P)) && (j & (1 << C))) std::swap(w[j], w[j | (1 << P)]);
+                } else if (arm < mlx::ARM_DIAG11) {   // diag(d0, d1) on P
+                    const int P = (arm - mlx::ARM_DIAG1) % 4; const bool only1 = arm >= mlx::ARM_DIAG1_ONLY1;
+                    if (op.kind != mlx::BOP_DIAG) { printf("DIAG1 arm on a non-diagonal op\n"); exit(1); }
+                    for (int j = 0; j < NV; j++) {
+                        if (j & (1 << P)) w[j] *= cd(op.m[2], op.m[3]);
+                        else if (!only1) w[j] *= cd(op.m[0], op.m[1]);
+                    }
+                } else if (arm < mlx::ARM_DIAG) {   // diag(1, 1, 1, d3) on P < Q
+                    const int P = (arm - mlx::ARM_DIAG11) / 4, Q = (arm - mlx::ARM_DIAG11) % 4;
+                    if (op.kind != mlx::BOP_DIAG || P >= Q) { printf("DIAG11 arm mismatch\n"); exit(1); }
+                    for (int j = 0; j < NV; j++) if ((j & (1 << P)) && (j & (1 << Q))) w[j] *= cd(op.m[6], op.m[7]);
+                } else if (arm == mlx::ARM_DIAG) {
+                    if (op.kind != mlx::BOP_DIAG) { printf("DIAG arm on a non-diagonal op\n"); exit(1); }
                     auto gbit = [&](int32_t c) { return (c < 0 && c != INT32_MIN) ? (int)((base >> (-1 - c)) & 1) : 0; };
                     const int kg = gbit(op.lq) | (gbit(op.lq2) << 1);
                     for (int j = 0; j < NV; j++) {
@@ -134,7 +151,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
                         const int kk = kg | (int)((op.cl >> (2 * j)) & 3u);
                         w[j] *= cd(op.m[2 * kk], op.m[2 * kk + 1]);
                     }
-                } else { printf("op kind %d in a register phase\n", op.kind); exit(1); }
+                } else { printf("unknown arm %d\n", arm); exit(1); }
             }
         }
         k += 1 + h.lq;
