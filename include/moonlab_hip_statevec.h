@@ -175,14 +175,17 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_sta
                                                                  const moonlab_hip_gate_t *gates,
                                                                  int num_gates);
 /* Addition: how many passes over the state moonlab_hip_apply_gates_blocked
- * makes for this gate list on n qubits (host only, no device needed), or a
- * negative moonlab_hip_status_t value. */
-MOONLAB_API int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *gates, int num_gates, int n);
+ * makes for this gate list on an n-qubit state of the given precision (host
+ * only, no device needed), or a negative moonlab_hip_status_t value. */
+MOONLAB_API int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *gates, int num_gates, int n,
+                                               int precision);
 /* Addition: tile (11, 12 or 13 qubits; the register-phase engine uses 12 or
- * 13, the LDS op engine 11 or 12) and the LDS op engine's workgroup size (256
- * or 512) of the blocked path; 0 keeps a value.  Process-wide, like the gate
- * tuning.  Different tiles may reorder commuting gates, so results agree to
- * rounding; one setting gives the same result on every call. */
+ * 13 -- by default 12 for complex128 and 13 for complex64 -- the LDS op engine
+ * 11 or 12) and the LDS op engine's workgroup size (256 or 512) of the
+ * blocked path; 0 keeps a value, tile_bits -1 restores the default by
+ * precision.  Process-wide, like the gate tuning.  Different tiles may
+ * reorder commuting gates, so results agree to rounding; one setting gives
+ * the same result on every call. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads);
 /* Addition: engine of the blocked path for states of at least 12 qubits:
  * 0 = register phases (default: each thread keeps 16 tile amplitudes in

@@ -187,8 +187,8 @@ typedef struct {
     int      batch_engines;      /* concurrent batch launches (default 2) */
     int      large_workers;      /* concurrent large requests, each on its own stream (default 2) */
     int      sequential;         /* 1: plain dispatch -- one request at a time, one kernel per gate */
-    int      large_path;         /* large requests: 0 fused <= 5-qubit blocks on MFMA (default),
-                                    1 cache-blocked passes */
+    int      large_path;         /* large requests: 0 cache-blocked register-phase passes
+                                    (default), 1 fused <= 5-qubit blocks on MFMA */
 } moonlab_mi355x_server_config_t;
 
 typedef struct {

@@ -300,7 +300,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_fusion_plan": ([vp, ip, ip, u64p, ctypes.POINTER(ctypes.c_int), dp, ip], ip),
         "moonlab_hip_apply_gates_blocked": ([vp, vp, ip], ip),
         "moonlab_hip_expect_pauli": ([vp, u64, u64, dp], ip),
-        "moonlab_hip_blocked_pass_count": ([vp, ip, ip], ip),
+        "moonlab_hip_blocked_pass_count": ([vp, ip, ip, ip], ip),
         "moonlab_hip_set_blocked_tuning": ([ip, ip], ip),
         "moonlab_hip_set_blocked_engine": ([ip], ip),
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
@@ -819,18 +819,18 @@ def fusion_plan(gates, kmax: int = 5) -> List[FusedBlock]:
     return out
 
 
-def blocked_pass_count(gates, n: int) -> int:
+def blocked_pass_count(gates, n: int, precision: int = C128) -> int:
     """HBM passes StateVector.apply(gates, blocked=True) makes on n qubits."""
     g = gate_array(gates)
-    k = lib().moonlab_hip_blocked_pass_count(g.ctypes.data, len(g), n)
+    k = lib().moonlab_hip_blocked_pass_count(g.ctypes.data, len(g), n, precision)
     if k < 0:
         raise EngineError(f"blocked_pass_count failed ({k})")
     return k
 
 
 def set_blocked_tuning(tile_bits: int = 0, threads: int = 0) -> None:
-    """Tile (11, 12 or 13 qubits) and LDS-engine workgroup size (256 or 512)
-    of the cache-blocked path; 0 keeps a value."""
+    """Tile (11, 12 or 13 qubits; -1: by precision, the default) and LDS-engine
+    workgroup size (256 or 512) of the cache-blocked path; 0 keeps a value."""
     _check_hip(lib().moonlab_hip_set_blocked_tuning(tile_bits, threads), "set_blocked_tuning")
 
 
@@ -1005,11 +1005,11 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int = 0, batch_max_qubits: int = 0, max_batch: int = 0,
                  batch_engines: int = 0, large_workers: int = 0, sequential: bool = False,
-                 large_path: str = "fused") -> None:
+                 large_path: str = "blocked") -> None:
         if large_path not in ("fused", "blocked"):
-            raise ValueError("large_path is 'fused' or 'blocked'")
+            raise ValueError("large_path is 'blocked' or 'fused'")
         cfg = _ServerConfig(hbm_budget_bytes, batch_max_qubits, max_batch, batch_engines, large_workers,
-                            int(sequential), 1 if large_path == "blocked" else 0)
+                            int(sequential), 1 if large_path == "fused" else 0)
         self._h = lib().moonlab_mi355x_server_create(ctypes.byref(cfg))
         if not self._h:
             raise EngineError("moonlab_mi355x_server_create failed (no HIP device?)")

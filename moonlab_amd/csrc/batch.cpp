@@ -389,13 +389,13 @@ uint64_t touches(const GOp &o)
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
-                 std::vector<BOp> &ops)
+                 std::vector<BOp> &ops, int anchor_bits)
 {
     const bool resident = n <= tb;
-    // low qubits kept in every tile (256-byte runs at complex128); at most
-    // tb - 2, so any op (two non-diagonal targets at most) fits an empty pass
-    // and every scan below schedules at least one op
-    const int anchor = resident ? n : (tb - 2 < 4 ? (tb - 2 > 0 ? tb - 2 : 0) : 4);
+    // low qubits kept in every tile (runs of 2^anchor_bits contiguous
+    // amplitudes); at most tb - 2, so any op (two non-diagonal targets at
+    // most) fits an empty pass and every scan below schedules at least one op
+    const int anchor = resident ? n : (tb - 2 < anchor_bits ? (tb - 2 > 0 ? tb - 2 : 0) : anchor_bits);
     uint64_t anchor_mask = anchor >= 64 ? ~0ull : ((1ull << anchor) - 1);
     std::vector<const GOp *> cur;
     uint64_t S = anchor_mask;
@@ -474,13 +474,21 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
 namespace {
 
 // Tile bits and engine of the single-state blocked path: the register-phase
-// engine for states of at least 12 qubits (12- or 13-qubit tiles), the LDS
-// op engine otherwise or when selected (11- or 12-qubit tiles).
-int blocked_shape(int n, const GateTuning &t, bool *reg)
+// engine for states of at least 12 qubits (12-qubit tiles for complex128,
+// 13 for complex64 unless the tuning fixes one), the LDS op engine otherwise
+// or when selected (11- or 12-qubit tiles).  *anchor: the low qubits every
+// tile keeps, 128 contiguous bytes (3 qubits at complex128, 4 at complex64;
+// the LDS engine keeps 4).
+int blocked_shape(int n, int prec, const GateTuning &t, bool *reg, int *anchor)
 {
     *reg = t.blocked_engine == 0 && n >= 12;
-    if (*reg) return (t.blocked_tile_bits >= 13 && n >= 13) ? 13 : 12;
-    const int b = t.blocked_tile_bits > 12 ? 12 : t.blocked_tile_bits;
+    if (*reg) {
+        *anchor = prec == MOONLAB_HIP_C64 ? 4 : 3;
+        const int want = t.blocked_tile_bits ? t.blocked_tile_bits : (prec == MOONLAB_HIP_C64 ? 13 : 12);
+        return (want >= 13 && n >= 13) ? 13 : 12;
+    }
+    *anchor = 4;
+    const int b = t.blocked_tile_bits == 0 || t.blocked_tile_bits > 12 ? 12 : t.blocked_tile_bits;
     return n < b ? n : b;
 }
 
@@ -811,9 +819,11 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     std::vector<BPass> passes;
     std::vector<BOp> ops;
     const GateTuning tune = gate_tuning();
+    const int prec = moonlab_hip_state_precision(s);
     bool reg = false;
-    const int tb = blocked_shape(n, tune, &reg);
-    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops);
+    int anchor = 4;
+    const int tb = blocked_shape(n, prec, tune, &reg, &anchor);
+    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
     static std::atomic<int> cached_cus{0};   // shared by every launching thread
@@ -829,9 +839,8 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     hipError_t e = hipMallocAsync((void **)&d_ops, ops.size() * sizeof(BOp), st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);   // `ops` is a local vector
-    const int prec = moonlab_hip_state_precision(s);
     void *amps = moonlab_hip_state_device_ptr(s);
-    const int lds_tile = tune.blocked_tile_bits > 12 ? 12 : tune.blocked_tile_bits;
+    const int lds_tile = tune.blocked_tile_bits == 0 || tune.blocked_tile_bits > 12 ? 12 : tune.blocked_tile_bits;
     for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
         e = reg ? launch_reg_pass(amps, prec, n, passes[i], d_ops, cus, st)
                 : launch_tile_pass(amps, prec, n, passes[i], d_ops, cus, tune.blocked_threads, lds_tile, st);
@@ -840,16 +849,18 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     return e == hipSuccess ? MOONLAB_HIP_OK : MOONLAB_HIP_ERR_DRIVER;
 }
 
-int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n)
+int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n, int precision)
 {
     if (ng < 0 || (ng > 0 && !g) || n < 1 || n > 40) return MOONLAB_HIP_ERR_BAD_ARG;
+    if (precision != MOONLAB_HIP_C128 && precision != MOONLAB_HIP_C64) return MOONLAB_HIP_ERR_BAD_ARG;
     std::vector<GOp> gops;
     if (!lower(g, ng, n, gops)) return MOONLAB_HIP_ERR_INVALID_QUBIT;
     std::vector<BPass> passes;
     std::vector<BOp> ops;
     bool reg = false;
-    const int tb = blocked_shape(n, gate_tuning(), &reg);
-    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops);
+    int anchor = 4;
+    const int tb = blocked_shape(n, precision, gate_tuning(), &reg, &anchor);
+    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor);
     return (int)passes.size();
 }
 

@@ -24,11 +24,12 @@ struct GOp {
 bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out);
 
 // Pack ops into passes over tile sets of `tb` qubits (n <= tb: one resident
-// pass over all n qubits), convert qubits to tile-local indices and group
-// each pass's ops into register segments of `seg_regs` bits (BOP_SEG), or,
-// for seg_regs = kRegPhases, into a register-phase program (BOP_PHASE).
+// pass over all n qubits) that always hold the lowest `anchor_bits` qubits,
+// convert qubits to tile-local indices and group each pass's ops into
+// register segments of `seg_regs` bits (BOP_SEG), or, for seg_regs =
+// kRegPhases, into a register-phase program (BOP_PHASE).
 constexpr int kRegPhases = -1;
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
-                 std::vector<BOp> &ops);
+                 std::vector<BOp> &ops, int anchor_bits = 4);
 
 }  // namespace mlx

@@ -14,10 +14,10 @@
 //     max_batch) into ONE persistent batch launch; with two engines, one batch
 //     is built and launched while the other runs, so arrivals never wait for
 //     a whole batch to drain before getting on the GPU;
-//   * large requests run as fused <= 5-qubit blocks on the matrix cores (or
-//     cache-blocked passes, large_path = 1), each large worker on one stream
-//     of its own for its whole life, overlapping with the batches on the
-//     device;
+//   * large requests run as cache-blocked passes (register-phase tiles; or
+//     fused <= 5-qubit blocks on the matrix cores, large_path = 1), each large
+//     worker on one stream of its own for its whole life, overlapping with the
+//     batches on the device;
 //   * sequential mode is the plain baseline: one request at a time, one
 //     kernel per gate.
 // Results never depend on co-tenants: the batch engine's per-circuit planning
@@ -201,7 +201,7 @@ void large_thread(moonlab_mi355x_server_t *s)
             r->t_start = clock_s() - s->t0;
             s->stats.large_requests++;
         }
-        run_single(r, s->cfg.large_path == 1 ? 1 : 2, stream);
+        run_single(r, s->cfg.large_path == 1 ? 2 : 1, stream);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             r->path = 1;

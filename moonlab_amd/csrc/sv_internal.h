@@ -25,7 +25,7 @@ struct GateTuning {
     int ept;     // 1, 2, 4 or 8
     int block;   // 64, 128, 256, 512 or 1024
     uint64_t nt_min_bytes;   // states at least this large use streaming loads/stores
-    int blocked_tile_bits = 12;   // tile of the single-state blocked path: 11, 12 or 13 qubits
+    int blocked_tile_bits = 0;    // tile of the single-state blocked path: 11, 12 or 13 qubits (0: by precision)
     int blocked_threads = 512;    // workgroup size of its LDS op engine: 256 or 512
     int blocked_engine = 0;       // 0: register phases (k_reg_pass), 1: LDS ops (k_tile_pass)
 };

@@ -179,8 +179,9 @@ moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads)
 {
     GateTuning &t = gate_tuning();
     if (tile_bits) {
-        if (tile_bits < 11 || tile_bits > 13) return fail(MOONLAB_HIP_ERR_BAD_ARG, "tile_bits is 11, 12 or 13");
-        t.blocked_tile_bits = tile_bits;
+        if (tile_bits != -1 && (tile_bits < 11 || tile_bits > 13))
+            return fail(MOONLAB_HIP_ERR_BAD_ARG, "tile_bits is 11, 12, 13 or -1 (by precision)");
+        t.blocked_tile_bits = tile_bits == -1 ? 0 : tile_bits;
     }
     if (threads) {
         if (threads != 256 && threads != 512) return fail(MOONLAB_HIP_ERR_BAD_ARG, "threads is 256 or 512");

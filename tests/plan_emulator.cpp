@@ -214,7 +214,7 @@ int main() {
             if (tb > n) continue;
             std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
             std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
-            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops);
+            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1));
             std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
             for (const mlx::BPass &P : passes) {
                 if (P.nbits != tb) { printf("register pass with %d tile bits\n", P.nbits); return 1; }

@@ -494,7 +494,7 @@ def test_blocked_engines_vs_oracle(eng, oracle, engine, tile_bits):
             assert abs(np.vdot(want, got)) ** 2 > 1 - 1e-6, (engine, tile_bits, n)
     finally:
         eng.set_blocked_engine("register")
-        eng.set_blocked_tuning(12, 512)
+        eng.set_blocked_tuning(-1, 512)
 
 
 @pytest.mark.gpu
