@@ -331,6 +331,68 @@ def fusion_cpu_baseline(n: int, gates: np.ndarray, budget_s: float) -> dict | No
                       f"({dt:.1f} s; state init {t_init:.1f} s excluded), tsotchke/moonlab C built by oracle/Makefile"}
 
 
+def run_blocked_workload(m, n: int, depth: int) -> dict:
+    """The large-request engine: the n-qubit random-layer circuit (the
+    serving stream's circuit family; H layer first) run as cache-blocked
+    register-phase passes (k_reg_pass), beside the same circuit as fused
+    <= 5-qubit MFMA blocks and as one kernel per gate, per precision.  The
+    complex128 register-engine state is checked against the per-gate state
+    (fidelity, and max |amplitude| difference over 64 random runs of 4096
+    amplitudes)."""
+    from moonlab_amd import workloads
+    g = workloads.random_layer_circuit(n, depth, SEED + 7, h_layer=True)
+    D = 1 << n
+    out = {}
+    with m.StateVector(12, m.C64) as w:         # load the code objects outside the timing
+        w.apply(g[:40] if n == 12 else workloads.random_layer_circuit(12, 2, 1), blocked=True)
+        w.synchronize()
+
+    def timed(st, **kw):
+        st.timer_begin(2)
+        st.timer_mark()
+        st.apply(g, **kw)
+        st.timer_mark()
+        return float(st.timer_elapsed(1)[0]) * 1e-3
+
+    for prec, name in ((m.C128, "c128"), (m.C64, "c64")):
+        b = 8 if prec == m.C64 else 16
+        passes = m.blocked_pass_count(g, n, prec)
+        res = {"passes": passes, "passes_per_layer": round(passes / depth, 2)}
+        st = m.StateVector(n, prec)
+        t_reg = timed(st, blocked=True)
+        res["register_seconds"] = round(t_reg, 4)
+        res["register_ms_per_pass"] = round(t_reg / passes * 1e3, 3)
+        res["register_TB_per_s_per_pass"] = round(passes * 2 * D * b / t_reg / 1e12, 3)
+        if prec == m.C128:
+            ref = m.StateVector(n, prec)
+            t_gate = timed(ref)
+            res["per_gate_seconds"] = round(t_gate, 4)
+            _, n1 = st.expect_z_all()
+            _, n2 = ref.expect_z_all()
+            res["fidelity_vs_per_gate"] = abs(st.inner(ref)) ** 2 / (n1 * n2)
+            rng = np.random.default_rng(SEED)
+            run = min(D, 1 << 12)
+            err = 0.0
+            for off in rng.integers(0, D // run, size=64) * run:   # 64 random runs of 4096 amplitudes
+                a = st.read(int(off), run)
+                c = ref.read(int(off), run)
+                err = max(err, float(np.max(np.abs(a - c))))
+            res["max_abs_amp_diff_vs_per_gate_sampled"] = err
+            ref.free()
+        st.free()
+        st = m.StateVector(n, prec)
+        t_fused = timed(st, fuse=5)
+        st.free()
+        res["fused_seconds"] = round(t_fused, 4)
+        res["speedup_vs_fused"] = round(t_fused / t_reg, 3)
+        if "per_gate_seconds" in res:
+            res["speedup_vs_per_gate"] = round(res["per_gate_seconds"] / t_reg, 3)
+        print(f"[bench] blocked n={n} {name}: {passes} passes, register {t_reg:.3f} s, fused {t_fused:.3f} s",
+              file=sys.stderr, flush=True)
+        out[name] = res
+    return out
+
+
 def run_fusion_workload(m, depth: int, cases) -> dict:
     """Precision + fusion: depth-`depth` random SU(2) + CZ brickwork circuits,
     fused into <= 5-qubit blocks (composed in FP64) and contracted on the
@@ -417,7 +479,7 @@ def main() -> int:
     ap.add_argument("--serve-count", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=100.0)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--workload", choices=("sweep", "fusion"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "fusion", "blocked"), default="sweep",
                     help="sweep: the 30-qubit gate sweep (with the batch and serving legs); "
                          "fusion: 32q complex64 and 33q complex128 SU(2)+CZ depth 60 on fused MFMA blocks")
     ap.add_argument("--fusion-depth", type=int, default=60)
@@ -433,6 +495,27 @@ def main() -> int:
 
     import moonlab_amd as m
     from moonlab_amd import workloads
+
+    if args.workload == "blocked":
+        t0 = time.perf_counter()
+        depth = args.depth if args.depth != 40 else 10
+        res = run_blocked_workload(m, args.qubits, depth)
+        wall = time.perf_counter() - t0
+        c128 = res["c128"]
+        print(json.dumps({
+            "metric": "HBM TB/s per cache-blocked pass (register-phase engine, large-request path)",
+            "value": c128["register_TB_per_s_per_pass"], "unit": "TB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(wall * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64/f32", "data": "synthetic",
+            "config": {"workload": f"{args.qubits}-qubit random-layer circuit (H layer, depth {depth} of "
+                                   "RX|RY|RZ + CNOT brickwork), cache-blocked register-phase passes",
+                       "qubits": args.qubits, "depth": depth},
+            "roofline": {"bound": "hbm", "kernel": "k_reg_pass<double>",
+                         "achieved": round(c128["register_TB_per_s_per_pass"] * 1e3, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(c128["register_TB_per_s_per_pass"] * 1e3 / HBM_PEAK_GBS, 4),
+                         "traffic": None},
+            "cases": res}), flush=True)
+        return 0
 
     if args.workload == "fusion":
         cases = [(int(c.split(":")[0]), m.C64 if c.split(":")[1] == "c64" else m.C128)

@@ -25,7 +25,9 @@ GateTuning &gate_tuning()
 {
     // One pair per thread and 256-thread blocks measured fastest on MI355X for
     // every gate family at 30 qubits (2, 4, 8 pairs per thread: 2-10 % slower).
-    static GateTuning t = {1, 256, 512ull << 20, 12, 512};
+    // Blocked path: tile bits 0 = by precision (12 for complex128, 13 for
+    // complex64), 512-thread LDS-engine workgroups, register-phase engine.
+    static GateTuning t = {1, 256, 512ull << 20, 0, 512, 0};
     return t;
 }
 
