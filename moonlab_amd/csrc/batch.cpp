@@ -236,7 +236,7 @@ void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vecto
 // first phase tries the IO layout (the top kRegBits tile bits), which needs
 // no LDS exchange; a register set is filled up with the highest free bits.
 // SWAP lowers to three exchanges, X_b[a] X_a[b] X_b[a], its controls kept.
-void plan_reg_phases(const std::vector<BOp> &in0, int nbits, std::vector<BOp> &out)
+void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std::vector<BOp> &out)
 {
     std::vector<BOp> in;
     in.reserve(in0.size());
@@ -324,6 +324,13 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, std::vector<BOp> &o
         h.lq = (int32_t)cur.size();
         h.lq2 = (int32_t)S;
         for (int i = 0; i < kRegBits; i++) h.cl |= (uint32_t)r[i] << (5 * i);
+        // byte offsets of the register values' LDS slots in this layout
+        uint32_t slot_off[1 << kRegBits];
+        static_assert(sizeof slot_off <= sizeof h.m, "slot offsets fit the header's m[]");
+        for (int j = 0; j < (1 << kRegBits); j++)
+            slot_off[j] = (uint32_t)((amp_bytes == 8 ? lds_slot<8>((int)roff(j)) : lds_slot<16>((int)roff(j))) *
+                                     amp_bytes);
+        memcpy(h.m, slot_off, sizeof slot_off);
         out.push_back(h);
         for (const BOp *o : cur) {
             BOp b = *o;
@@ -389,7 +396,7 @@ uint64_t touches(const GOp &o)
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
-                 std::vector<BOp> &ops, int anchor_bits)
+                 std::vector<BOp> &ops, int anchor_bits, int amp_bytes)
 {
     const bool resident = n <= tb;
     // low qubits kept in every tile (runs of 2^anchor_bits contiguous
@@ -434,7 +441,7 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
             memcpy(b.m, o->m, sizeof b.m);
             pass_ops.push_back(b);
         }
-        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, ops);
+        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, amp_bytes, ops);
         else segment_ops(pass_ops, P.nbits, seg_regs, ops);
         P.num_ops = (int)ops.size() - P.first_op;
         passes.push_back(P);
@@ -823,7 +830,8 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     bool reg = false;
     int anchor = 4;
     const int tb = blocked_shape(n, prec, tune, &reg, &anchor);
-    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor);
+    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor,
+                prec == MOONLAB_HIP_C64 ? 8 : 16);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
     static std::atomic<int> cached_cus{0};   // shared by every launching thread
@@ -860,7 +868,8 @@ int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n, i
     bool reg = false;
     int anchor = 4;
     const int tb = blocked_shape(n, precision, gate_tuning(), &reg, &anchor);
-    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor);
+    plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor,
+                precision == MOONLAB_HIP_C64 ? 8 : 16);
     return (int)passes.size();
 }
 

@@ -20,7 +20,9 @@ enum BOpKind : int32_t {
                     // they run on 2^lq2 amplitudes held in registers per thread
     BOP_PHASE = 5,  // register-phase program header (k_reg_pass): the next lq ops run on the
                     // layout whose kRegBits register bits are the tile bits of mask lq2
-                    // (positions packed 5 bits each, ascending, in cl); op fields below
+                    // (positions packed 5 bits each, ascending, in cl); m[] holds the
+                    // 2^kRegBits uint32 byte offsets of the register values' LDS slots,
+                    // lds_slot(reg_off(j)) x amplitude bytes; op fields below
 };
 
 // Register-phase programs (k_reg_pass): each thread holds 2^kRegBits tile

@@ -36,21 +36,25 @@ __device__ __forceinline__ uint64_t tile_base_c(uint64_t t, const BPass &p)
 constexpr int RB = kRegBits, NV = 1 << RB;
 
 // A layout change: every thread writes its register values at their tile
-// indices (LDS slots), then reads the values of the new layout.
-
+// indices (LDS slots), then reads the values of the new layout.  from / to:
+// the byte offsets of the 16 register values' LDS slots in each layout
+// (slot(reg_off(L, j)) x amplitude bytes; the planner stores them in the
+// phase header), so the only per-value address work is one XOR with the
+// thread's own slot.
 template <typename R>
-__device__ __forceinline__ void relayout(cplx_t<R> *t, cplx_t<R> (&v)[NV], int &tloc, const RegLayout &from,
-                                         const RegLayout &to)
+__device__ __forceinline__ void relayout(cplx_t<R> *t, cplx_t<R> (&v)[NV], int &tloc, const uint32_t (&from)[NV],
+                                         const uint32_t (&to)[NV], const RegLayout &tol)
 {
     constexpr int B = (int)sizeof(cplx_t<R>);
-    const int s0 = lds_slot<B>(tloc);
+    char *tb = (char *)t;
+    const uint32_t s0 = (uint32_t)lds_slot<B>(tloc) * B;
 #pragma unroll
-    for (int j = 0; j < NV; j++) t[s0 ^ lds_slot<B>(reg_off(from, j))] = v[j];
+    for (int j = 0; j < NV; j++) *(cplx_t<R> *)(tb + (s0 ^ from[j])) = v[j];
     lds_barrier();
-    tloc = thread_base(to, (int)threadIdx.x);
-    const int s1 = lds_slot<B>(tloc);
+    tloc = thread_base(tol, (int)threadIdx.x);
+    const uint32_t s1 = (uint32_t)lds_slot<B>(tloc) * B;
 #pragma unroll
-    for (int j = 0; j < NV; j++) v[j] = t[s1 ^ lds_slot<B>(reg_off(to, j))];
+    for (int j = 0; j < NV; j++) v[j] = *(const cplx_t<R> *)(tb + (s1 ^ to[j]));
 }
 
 // 2x2 on register bit P for the register values j whose register-bit
@@ -257,10 +261,14 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restri
         }
     }
     __syncthreads();
+    constexpr int B = (int)sizeof(cplx_t<R>);
     RegLayout io;
 #pragma unroll
     for (int i = 0; i < RB; i++) io.r[i] = TB - RB + i;
     const uint32_t io_mask = ((1u << RB) - 1) << (TB - RB);
+    uint32_t io_off[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) io_off[j] = (uint32_t)lds_slot<B>(j << (TB - RB)) * B;
     // IO layout: value j of thread tid is tile index tid + THREADS j
     const uint64_t lo = dlo[tid & 63];
     const int hi0 = tid >> 6;
@@ -283,22 +291,28 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restri
         }
         const uint64_t base = tile_base_c<TB>(ti, P);
         int tloc = tid;
-        RegLayout cur = io;
+        uint32_t cur_off[NV];
+#pragma unroll
+        for (int j = 0; j < NV; j++) cur_off[j] = io_off[j];
         uint32_t cur_mask = io_mask;
         for (int k = 0; k < P.num_ops;) {
             const BOp &h = pops[k];   // BOP_PHASE header
             const uint32_t mask = (uint32_t)h.lq2;
             if (mask != cur_mask) {
-                const RegLayout nl = unpack_layout(h.cl);
-                relayout<R>(t, v, tloc, cur, nl);
-                cur = nl;
+                uint32_t nof[NV];
+                const uint32_t *hs = (const uint32_t *)h.m;   // the new layout's slot byte offsets
+#pragma unroll
+                for (int j = 0; j < NV; j++) nof[j] = hs[j];
+                relayout<R>(t, v, tloc, cur_off, nof, unpack_layout(h.cl));
+#pragma unroll
+                for (int j = 0; j < NV; j++) cur_off[j] = nof[j];
                 cur_mask = mask;
             }
             const int cnt = h.lq;
             for (int q = 1; q <= cnt; q++) reg_op<R>(v, pops[k + q], base);
             k += 1 + cnt;
         }
-        if (cur_mask != io_mask) relayout<R>(t, v, tloc, cur, io);
+        if (cur_mask != io_mask) relayout<R>(t, v, tloc, cur_off, io_off, io);
 #pragma unroll
         for (int j = 0; j < NV; j++) st<NT>(&a[(base | lo) | dhi[hi0 + HSTEP * j]], v[j]);
     }

@@ -81,13 +81,18 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
     mlx::RegLayout io; for (int i = 0; i < RB; i++) io.r[i] = TB - RB + i;
     const uint32_t io_mask = ((1u << RB) - 1) << (TB - RB);
     for (int tid = 0; tid < TH; tid++) { tloc[tid] = tid; for (int j = 0; j < NV; j++) v[tid][j] = tile[tid + TH * j]; }
-    auto relayout = [&](const mlx::RegLayout &from, const mlx::RegLayout &to) {
+    // from_off / to_off: byte offsets of the register values' slots, as the
+    // kernel gets them (computed for the IO layout, from the phase header
+    // otherwise)
+    auto relayout = [&](const uint32_t *from_off, const uint32_t *to_off, const mlx::RegLayout &to) {
         relayouts++;
         std::fill(written.begin(), written.end(), 0);
-        for (int tid = 0; tid < TH; tid++) { int s0 = mlx::lds_slot<BYTES>(tloc[tid]); for (int j = 0; j < NV; j++) { int sl = s0 ^ mlx::lds_slot<BYTES>(mlx::reg_off(from, j)); if (sl < 0 || sl >= (1 << TB)) { printf("slot out of range\n"); exit(1); } written[sl]++; lds[sl] = v[tid][j]; } }
+        for (int tid = 0; tid < TH; tid++) { uint32_t s0 = (uint32_t)mlx::lds_slot<BYTES>(tloc[tid]) * BYTES; for (int j = 0; j < NV; j++) { uint32_t b = s0 ^ from_off[j]; if (b % BYTES || b / BYTES >= (1u << TB)) { printf("slot out of range\n"); exit(1); } written[b / BYTES]++; lds[b / BYTES] = v[tid][j]; } }
         for (int x : written) if (x != 1) { printf("layout change does not write every slot once\n"); exit(1); }
-        for (int tid = 0; tid < TH; tid++) { tloc[tid] = mlx::thread_base(to, tid); int s1 = mlx::lds_slot<BYTES>(tloc[tid]); for (int j = 0; j < NV; j++) v[tid][j] = lds[s1 ^ mlx::lds_slot<BYTES>(mlx::reg_off(to, j))]; }
+        for (int tid = 0; tid < TH; tid++) { tloc[tid] = mlx::thread_base(to, tid); uint32_t s1 = (uint32_t)mlx::lds_slot<BYTES>(tloc[tid]) * BYTES; for (int j = 0; j < NV; j++) v[tid][j] = lds[(s1 ^ to_off[j]) / BYTES]; }
     };
+    uint32_t io_off[16], cur_off[16];
+    for (int j = 0; j < NV; j++) cur_off[j] = io_off[j] = (uint32_t)mlx::lds_slot<BYTES>(j << (TB - RB)) * BYTES;
     mlx::RegLayout cur = io; uint32_t cur_mask = io_mask;
     for (int k = 0; k < nops;) {
         const mlx::BOp &h = ops[k];
@@ -96,7 +101,10 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
         mlx::RegLayout nl = mlx::unpack_layout(h.cl);
         uint32_t m2 = 0; for (int i = 0; i < RB; i++) { m2 |= 1u << nl.r[i]; if (i && nl.r[i] <= nl.r[i - 1]) { printf("phase bits not ascending\n"); exit(1); } if (nl.r[i] >= TB) { printf("phase bit outside tile\n"); exit(1); } }
         if (m2 != mask) { printf("phase mask does not match its bits\n"); exit(1); }
-        if (mask != cur_mask) { relayout(cur, nl); cur = nl; cur_mask = mask; }
+        if (mask != cur_mask) {
+            uint32_t nof[16]; memcpy(nof, h.m, sizeof nof);
+            relayout(cur_off, nof, nl); memcpy(cur_off, nof, sizeof nof); cur = nl; cur_mask = mask;
+        }
         for (int q = 1; q <= h.lq; q++) {
             const mlx::BOp &op = ops[k + q];
             if ((base & op.cg) != op.cg) continue;
@@ -156,7 +164,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
         }
         k += 1 + h.lq;
     }
-    if (cur_mask != io_mask) relayout(cur, io);
+    if (cur_mask != io_mask) relayout(cur_off, io_off, io);
     for (int tid = 0; tid < TH; tid++) for (int j = 0; j < NV; j++) tile[tid + TH * j] = v[tid][j];
 }
 int main() {
@@ -214,7 +222,7 @@ int main() {
             if (tb > n) continue;
             std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
             std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
-            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1));
+            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1), (trial & 1) ? 8 : 16);
             std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
             for (const mlx::BPass &P : passes) {
                 if (P.nbits != tb) { printf("register pass with %d tile bits\n", P.nbits); return 1; }
