@@ -190,7 +190,9 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, i
 /* Addition: engine of the blocked path for states of at least 12 qubits:
  * 0 = register phases (default: each thread keeps 16 tile amplitudes in
  * registers, LDS only changes which tile qubits those are), 1 = LDS ops (one
- * LDS sweep per op or register segment).  Smaller states use the LDS engine. */
+ * LDS sweep per op or register segment), 2 = register phases (same as 0),
+ * 3 = register phases with 8 amplitudes per thread (twice the waves, three
+ * register qubits per phase).  Smaller states use the LDS engine. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_set_blocked_engine(int engine);
 
 /* ---- Reductions ---------------------------------------------------------- */

@@ -836,9 +836,10 @@ def set_blocked_tuning(tile_bits: int = 0, threads: int = 0) -> None:
 
 def set_blocked_engine(engine: str) -> None:
     """Engine of the cache-blocked path for states of >= 12 qubits:
-    "register" (each thread keeps 16 tile amplitudes in registers; default)
+    "register" (each thread keeps 16 tile amplitudes in registers; default;
+    "register16" is the same), "register8" (8 per thread, twice the waves)
     or "lds" (one LDS sweep per op or register segment)."""
-    code = {"register": 0, "lds": 1}[engine]
+    code = {"register": 0, "lds": 1, "register16": 2, "register8": 3}[engine]
     _check_hip(lib().moonlab_hip_set_blocked_engine(code), "set_blocked_engine")
 
 

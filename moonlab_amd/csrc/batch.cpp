@@ -38,7 +38,8 @@ namespace mlx {
 
 hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int threads,
                             int tile_bits, hipStream_t st);
-hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, hipStream_t st);
+hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int reg_bits,
+                           hipStream_t st);
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
@@ -228,16 +229,17 @@ void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vecto
 }
 
 // A pass's tile-local ops as a register-phase program for k_reg_pass.  Each
-// phase has kRegBits register bits and takes, in order, every op whose
+// phase has reg_bits register bits and takes, in order, every op whose
 // non-diagonal target fits its register set and that shares no tile bit with
 // an op deferred from the phase (ops on disjoint qubits commute); the
 // deferred ops, still in order, make up the rest.  Every tile qubit an op
 // touches goes in the register set, so ops never depend on thread bits.  The
-// first phase tries the IO layout (the top kRegBits tile bits), which needs
+// first phase tries the IO layout (the top reg_bits tile bits), which needs
 // no LDS exchange; a register set is filled up with the highest free bits.
 // SWAP lowers to three exchanges, X_b[a] X_a[b] X_b[a], its controls kept.
-void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std::vector<BOp> &out)
+void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int reg_bits, std::vector<BOp> &out)
 {
+    const int nv = 1 << reg_bits;
     std::vector<BOp> in;
     in.reserve(in0.size());
     for (const BOp &b : in0) {
@@ -270,7 +272,7 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std:
         return m;
     };
     auto touch = need;
-    const uint32_t io = ((1u << kRegBits) - 1) << (nbits - kRegBits);
+    const uint32_t io = ((1u << reg_bits) - 1) << (nbits - reg_bits);
     std::vector<const BOp *> rest, later, cur;
     rest.reserve(in.size());
     for (const BOp &b : in) rest.push_back(&b);
@@ -283,7 +285,7 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std:
             later.clear();
             for (const BOp *o : rest) {
                 const uint32_t t = touch(*o), nd = need(*o);
-                const bool fits = fixed ? (nd & ~S) == 0 : __builtin_popcount(S | nd) <= kRegBits;
+                const bool fits = fixed ? (nd & ~S) == 0 : __builtin_popcount(S | nd) <= reg_bits;
                 if (!(t & blocked) && fits) {
                     S |= nd;
                     cur.push_back(o);
@@ -304,18 +306,18 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std:
         } else {
             scan(false);
         }
-        for (int q = nbits - 1; q >= 0 && __builtin_popcount(S) < kRegBits; q--) S |= 1u << q;
+        for (int q = nbits - 1; q >= 0 && __builtin_popcount(S) < reg_bits; q--) S |= 1u << q;
         int r[kRegBits];
         for (int q = 0, k = 0; q < nbits; q++)
             if (S & (1u << q)) r[k++] = q;
         auto roff = [&](int j) {
             uint32_t o = 0;
-            for (int i = 0; i < kRegBits; i++)
+            for (int i = 0; i < reg_bits; i++)
                 if ((j >> i) & 1) o |= 1u << r[i];
             return o;
         };
         auto pos = [&](int q) {
-            for (int i = 0; i < kRegBits; i++)
+            for (int i = 0; i < reg_bits; i++)
                 if (r[i] == q) return i;
             return 15;
         };
@@ -323,11 +325,11 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std:
         h.kind = BOP_PHASE;
         h.lq = (int32_t)cur.size();
         h.lq2 = (int32_t)S;
-        for (int i = 0; i < kRegBits; i++) h.cl |= (uint32_t)r[i] << (5 * i);
+        for (int i = 0; i < reg_bits; i++) h.cl |= (uint32_t)r[i] << (5 * i);
         // byte offsets of the register values' LDS slots in this layout
-        uint32_t slot_off[1 << kRegBits];
+        uint32_t slot_off[1 << kRegBits] = {};
         static_assert(sizeof slot_off <= sizeof h.m, "slot offsets fit the header's m[]");
-        for (int j = 0; j < (1 << kRegBits); j++)
+        for (int j = 0; j < nv; j++)
             slot_off[j] = (uint32_t)((amp_bytes == 8 ? lds_slot<8>((int)roff(j)) : lds_slot<16>((int)roff(j))) *
                                      amp_bytes);
         memcpy(h.m, slot_off, sizeof slot_off);
@@ -336,12 +338,12 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std:
             BOp b = *o;
             const uint32_t creg = o->cl & S;
             uint32_t jm = 0;
-            for (int j = 0; j < (1 << kRegBits); j++)
+            for (int j = 0; j < nv; j++)
                 if ((roff(j) & creg) == creg) jm |= 1u << j;
             b.cl = 0;   // every local control is a register bit
             b.pad2 = jm;
             b.pad = 0;
-            const bool all = jm == (1u << (1 << kRegBits)) - 1;
+            const bool all = jm == (nv == 32 ? ~0u : (1u << nv) - 1);
             auto one = [](const double *d) { return d[0] == 1.0 && d[1] == 0.0; };
             if (o->kind == BOP_DIAG) {
                 // kmap: the local qubits' bits of each register value; with
@@ -350,7 +352,7 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, std:
                 const int apos = o->lq >= 0 ? pos(o->lq) : -1, bpos = o->lq2 >= 0 ? pos(o->lq2) : -1;
                 const bool global = (o->lq < 0 && o->lq != INT32_MIN) || (o->lq2 < 0 && o->lq2 != INT32_MIN);
                 uint32_t kmap = 0;
-                for (int j = 0; j < (1 << kRegBits); j++) {
+                for (int j = 0; j < nv; j++) {
                     const uint32_t k = (apos >= 0 ? (j >> apos) & 1 : 0) | (bpos >= 0 ? ((j >> bpos) & 1) << 1 : 0);
                     kmap |= k << (2 * j);
                     if (!global && one(o->m + 2 * k)) b.pad2 &= ~(1u << j);
@@ -396,7 +398,7 @@ uint64_t touches(const GOp &o)
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
-                 std::vector<BOp> &ops, int anchor_bits, int amp_bytes)
+                 std::vector<BOp> &ops, int anchor_bits, int amp_bytes, int reg_bits)
 {
     const bool resident = n <= tb;
     // low qubits kept in every tile (runs of 2^anchor_bits contiguous
@@ -441,7 +443,7 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
             memcpy(b.m, o->m, sizeof b.m);
             pass_ops.push_back(b);
         }
-        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, amp_bytes, ops);
+        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, amp_bytes, reg_bits, ops);
         else segment_ops(pass_ops, P.nbits, seg_regs, ops);
         P.num_ops = (int)ops.size() - P.first_op;
         passes.push_back(P);
@@ -481,17 +483,21 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
 namespace {
 
 // Tile bits and engine of the single-state blocked path: the register-phase
-// engine for states of at least 12 qubits (12-qubit tiles for complex128,
-// 13 for complex64 unless the tuning fixes one), the LDS op engine otherwise
-// or when selected (11- or 12-qubit tiles).  *anchor: the low qubits every
-// tile keeps, 128 contiguous bytes (3 qubits at complex128, 4 at complex64;
-// the LDS engine keeps 4).
-int blocked_shape(int n, int prec, const GateTuning &t, bool *reg, int *anchor)
+// engine for states of at least 12 qubits (16 register values per thread;
+// 12-qubit tiles for complex128, 13-qubit tiles for complex64, unless the
+// tuning fixes them), the LDS op engine otherwise or when selected (11- or
+// 12-qubit tiles).  *anchor: the low qubits every tile
+// keeps, 128 contiguous bytes (3 qubits at complex128, 4 at complex64; the
+// LDS engine keeps 4); *reg_bits: register bits of the register engine.
+int blocked_shape(int n, int prec, const GateTuning &t, bool *reg, int *anchor, int *reg_bits)
 {
-    *reg = t.blocked_engine == 0 && n >= 12;
+    *reg = t.blocked_engine != 1 && n >= 12;
+    *reg_bits = kRegBits;
     if (*reg) {
-        *anchor = prec == MOONLAB_HIP_C64 ? 4 : 3;
-        const int want = t.blocked_tile_bits ? t.blocked_tile_bits : (prec == MOONLAB_HIP_C64 ? 13 : 12);
+        const bool c64 = prec == MOONLAB_HIP_C64;
+        *anchor = c64 ? 4 : 3;
+        *reg_bits = t.blocked_engine == 3 ? 3 : 4;
+        const int want = t.blocked_tile_bits ? t.blocked_tile_bits : (c64 ? 13 : 12);
         return (want >= 13 && n >= 13) ? 13 : 12;
     }
     *anchor = 4;
@@ -828,10 +834,10 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     const GateTuning tune = gate_tuning();
     const int prec = moonlab_hip_state_precision(s);
     bool reg = false;
-    int anchor = 4;
-    const int tb = blocked_shape(n, prec, tune, &reg, &anchor);
+    int anchor = 4, reg_bits = kRegBits;
+    const int tb = blocked_shape(n, prec, tune, &reg, &anchor, &reg_bits);
     plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor,
-                prec == MOONLAB_HIP_C64 ? 8 : 16);
+                prec == MOONLAB_HIP_C64 ? 8 : 16, reg_bits);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
     static std::atomic<int> cached_cus{0};   // shared by every launching thread
@@ -850,7 +856,7 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     void *amps = moonlab_hip_state_device_ptr(s);
     const int lds_tile = tune.blocked_tile_bits == 0 || tune.blocked_tile_bits > 12 ? 12 : tune.blocked_tile_bits;
     for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
-        e = reg ? launch_reg_pass(amps, prec, n, passes[i], d_ops, cus, st)
+        e = reg ? launch_reg_pass(amps, prec, n, passes[i], d_ops, cus, reg_bits, st)
                 : launch_tile_pass(amps, prec, n, passes[i], d_ops, cus, tune.blocked_threads, lds_tile, st);
     if (d_ops) hipFreeAsync(d_ops, st);
     if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
@@ -866,10 +872,10 @@ int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n, i
     std::vector<BPass> passes;
     std::vector<BOp> ops;
     bool reg = false;
-    int anchor = 4;
-    const int tb = blocked_shape(n, precision, gate_tuning(), &reg, &anchor);
+    int anchor = 4, reg_bits = kRegBits;
+    const int tb = blocked_shape(n, precision, gate_tuning(), &reg, &anchor, &reg_bits);
     plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor,
-                precision == MOONLAB_HIP_C64 ? 8 : 16);
+                precision == MOONLAB_HIP_C64 ? 8 : 16, reg_bits);
     return (int)passes.size();
 }
 

@@ -41,7 +41,9 @@ enum BOpKind : int32_t {
 //     (factor index kg | those bits); pad2 as above, less the values whose
 //     factor is exactly 1 when no qubit is global.
 //   SWAP does not occur (it is lowered to three exchanges).
-constexpr int kRegBits = 4;
+// A program is planned for one register width rb (3 or 4 bits: 8 or 16
+// values per thread); positions and masks above are over rb bits.
+constexpr int kRegBits = 4;   // the widest register set
 
 // Arms of the register-phase interpreter (P, Q, C = register positions).
 enum RegArm : int32_t {
@@ -62,28 +64,28 @@ struct RegLayout {
     int r[kRegBits];
 };
 
-__host__ __device__ inline RegLayout unpack_layout(uint32_t packed)
+__host__ __device__ inline RegLayout unpack_layout(uint32_t packed, int rb = kRegBits)
 {
-    RegLayout L;
-    for (int i = 0; i < kRegBits; i++) L.r[i] = (int)((packed >> (5 * i)) & 31u);
+    RegLayout L{};
+    for (int i = 0; i < rb; i++) L.r[i] = (int)((packed >> (5 * i)) & 31u);
     return L;
 }
 
 // tile-index bits of register value j
-__host__ __device__ inline int reg_off(const RegLayout &L, int j)
+__host__ __device__ inline int reg_off(const RegLayout &L, int j, int rb = kRegBits)
 {
     int o = 0;
-    for (int i = 0; i < kRegBits; i++)
+    for (int i = 0; i < rb; i++)
         if ((j >> i) & 1) o |= 1 << L.r[i];
     return o;
 }
 
 // tile index of a thread's register value 0: the thread id spread over the
 // non-register bits
-__host__ __device__ inline int thread_base(const RegLayout &L, int tid)
+__host__ __device__ inline int thread_base(const RegLayout &L, int tid, int rb = kRegBits)
 {
     int x = tid;
-    for (int i = 0; i < kRegBits; i++) x = ((x >> L.r[i]) << (L.r[i] + 1)) | (x & ((1 << L.r[i]) - 1));
+    for (int i = 0; i < rb; i++) x = ((x >> L.r[i]) << (L.r[i] + 1)) | (x & ((1 << L.r[i]) - 1));
     return x;
 }
 

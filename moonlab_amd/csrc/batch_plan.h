@@ -27,11 +27,11 @@ bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out);
 // pass over all n qubits) that always hold the lowest `anchor_bits` qubits,
 // convert qubits to tile-local indices and group each pass's ops into
 // register segments of `seg_regs` bits (BOP_SEG), or, for seg_regs =
-// kRegPhases, into a register-phase program (BOP_PHASE) for amplitudes of
-// `amp_bytes` bytes (16 complex128, 8 complex64; the phase headers carry LDS
-// slot offsets).
+// kRegPhases, into a register-phase program (BOP_PHASE) of `reg_bits`
+// register bits per phase for amplitudes of `amp_bytes` bytes (16
+// complex128, 8 complex64; the phase headers carry LDS slot offsets).
 constexpr int kRegPhases = -1;
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
-                 std::vector<BOp> &ops, int anchor_bits = 4, int amp_bytes = 16);
+                 std::vector<BOp> &ops, int anchor_bits = 4, int amp_bytes = 16, int reg_bits = kRegBits);
 
 }  // namespace mlx

@@ -27,7 +27,8 @@ struct GateTuning {
     uint64_t nt_min_bytes;   // states at least this large use streaming loads/stores
     int blocked_tile_bits = 0;    // tile of the single-state blocked path: 11, 12 or 13 qubits (0: by precision)
     int blocked_threads = 512;    // workgroup size of its LDS op engine: 256 or 512
-    int blocked_engine = 0;       // 0: register phases (k_reg_pass), 1: LDS ops (k_tile_pass)
+    int blocked_engine = 0;       // 0, 2: register phases (k_reg_pass), 16 values per thread; 1: LDS
+                                  // ops (k_tile_pass); 3: register phases of 8 values per thread
 };
 GateTuning &gate_tuning();
 

@@ -192,7 +192,7 @@ moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads)
 
 moonlab_hip_status_t moonlab_hip_set_blocked_engine(int engine)
 {
-    if (engine != 0 && engine != 1) return fail(MOONLAB_HIP_ERR_BAD_ARG, "engine is 0 or 1");
+    if (engine < 0 || engine > 3) return fail(MOONLAB_HIP_ERR_BAD_ARG, "engine is 0, 1, 2 or 3");
     gate_tuning().blocked_engine = engine;
     return MOONLAB_HIP_OK;
 }

@@ -72,13 +72,13 @@ static void apply_segment(std::vector<cd> &t, int T, const mlx::BOp *ops, int no
 // values, LDS slots swizzled for BYTES-byte amplitudes, a layout change =
 // every thread writes, then every thread reads.
 template <int BYTES>
-static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nops, uint64_t base, int &relayouts) {
-    const int RB = mlx::kRegBits, NV = 1 << RB, TH = 1 << (TB - RB);
+static void reg_tile(std::vector<cd> &tile, int TB, int RB, const mlx::BOp *ops, int nops, uint64_t base, int &relayouts) {
+    const int NV = 1 << RB, TH = 1 << (TB - RB);
     std::vector<cd> lds(1 << TB);
     std::vector<int> written(1 << TB);
     std::vector<std::vector<cd>> v(TH, std::vector<cd>(NV));
     std::vector<int> tloc(TH);
-    mlx::RegLayout io; for (int i = 0; i < RB; i++) io.r[i] = TB - RB + i;
+    mlx::RegLayout io{}; for (int i = 0; i < RB; i++) io.r[i] = TB - RB + i;
     const uint32_t io_mask = ((1u << RB) - 1) << (TB - RB);
     for (int tid = 0; tid < TH; tid++) { tloc[tid] = tid; for (int j = 0; j < NV; j++) v[tid][j] = tile[tid + TH * j]; }
     // from_off / to_off: byte offsets of the register values' slots, as the
@@ -89,7 +89,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
         std::fill(written.begin(), written.end(), 0);
         for (int tid = 0; tid < TH; tid++) { uint32_t s0 = (uint32_t)mlx::lds_slot<BYTES>(tloc[tid]) * BYTES; for (int j = 0; j < NV; j++) { uint32_t b = s0 ^ from_off[j]; if (b % BYTES || b / BYTES >= (1u << TB)) { printf("slot out of range\n"); exit(1); } written[b / BYTES]++; lds[b / BYTES] = v[tid][j]; } }
         for (int x : written) if (x != 1) { printf("layout change does not write every slot once\n"); exit(1); }
-        for (int tid = 0; tid < TH; tid++) { tloc[tid] = mlx::thread_base(to, tid); uint32_t s1 = (uint32_t)mlx::lds_slot<BYTES>(tloc[tid]) * BYTES; for (int j = 0; j < NV; j++) v[tid][j] = lds[(s1 ^ to_off[j]) / BYTES]; }
+        for (int tid = 0; tid < TH; tid++) { tloc[tid] = mlx::thread_base(to, tid, RB); uint32_t s1 = (uint32_t)mlx::lds_slot<BYTES>(tloc[tid]) * BYTES; for (int j = 0; j < NV; j++) v[tid][j] = lds[(s1 ^ to_off[j]) / BYTES]; }
     };
     uint32_t io_off[16], cur_off[16];
     for (int j = 0; j < NV; j++) cur_off[j] = io_off[j] = (uint32_t)mlx::lds_slot<BYTES>(j << (TB - RB)) * BYTES;
@@ -98,7 +98,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
         const mlx::BOp &h = ops[k];
         if (h.kind != mlx::BOP_PHASE || h.lq < 1) { printf("bad phase header\n"); exit(1); }
         const uint32_t mask = (uint32_t)h.lq2;
-        mlx::RegLayout nl = mlx::unpack_layout(h.cl);
+        mlx::RegLayout nl = mlx::unpack_layout(h.cl, RB);
         uint32_t m2 = 0; for (int i = 0; i < RB; i++) { m2 |= 1u << nl.r[i]; if (i && nl.r[i] <= nl.r[i - 1]) { printf("phase bits not ascending\n"); exit(1); } if (nl.r[i] >= TB) { printf("phase bit outside tile\n"); exit(1); } }
         if (m2 != mask) { printf("phase mask does not match its bits\n"); exit(1); }
         if (mask != cur_mask) {
@@ -117,7 +117,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
                 if (arm < mlx::ARM_XPERM) {   // MAT1 arms
                     const int P = arm % 4, form = arm < mlx::ARM_MAT1 ? 0 : (arm - mlx::ARM_MAT1) / 4;
                     const bool masked = arm < mlx::ARM_MAT1;
-                    if (op.kind != mlx::BOP_MAT1 || P != op.lq) { printf("MAT1 arm mismatch\n"); exit(1); }
+                    if (op.kind != mlx::BOP_MAT1 || P != op.lq || P >= RB) { printf("MAT1 arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) {
                         if (j & (1 << P)) continue;
                         if (masked && ((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
@@ -129,7 +129,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
                     }
                 } else if (arm < mlx::ARM_CX) {   // XPERM on jm
                     const int P = arm - mlx::ARM_XPERM;
-                    if (op.kind != mlx::BOP_XPERM || P != op.lq) { printf("XPERM arm mismatch\n"); exit(1); }
+                    if (op.kind != mlx::BOP_XPERM || P != op.lq || P >= RB) { printf("XPERM arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) {
                         if (j & (1 << P)) continue;
                         if (((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
@@ -137,18 +137,18 @@ static void reg_tile(std::vector<cd> &tile, int TB, const mlx::BOp *ops, int nop
                     }
                 } else if (arm < mlx::ARM_DIAG1) {   // CNOT, target P, control C
                     const int P = (arm - mlx::ARM_CX) / 4, C = (arm - mlx::ARM_CX) % 4;
-                    if (op.kind != mlx::BOP_XPERM || P != op.lq || P == C) { printf("CNOT arm mismatch\n"); exit(1); }
+                    if (op.kind != mlx::BOP_XPERM || P != op.lq || P == C || P >= RB || C >= RB) { printf("CNOT arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) if (!(j & (1 << P)) && (j & (1 << C))) std::swap(w[j], w[j | (1 << P)]);
                 } else if (arm < mlx::ARM_DIAG11) {   // diag(d0, d1) on P
                     const int P = (arm - mlx::ARM_DIAG1) % 4; const bool only1 = arm >= mlx::ARM_DIAG1_ONLY1;
-                    if (op.kind != mlx::BOP_DIAG) { printf("DIAG1 arm on a non-diagonal op\n"); exit(1); }
+                    if (op.kind != mlx::BOP_DIAG || P >= RB) { printf("DIAG1 arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) {
                         if (j & (1 << P)) w[j] *= cd(op.m[2], op.m[3]);
                         else if (!only1) w[j] *= cd(op.m[0], op.m[1]);
                     }
                 } else if (arm < mlx::ARM_DIAG) {   // diag(1, 1, 1, d3) on P < Q
                     const int P = (arm - mlx::ARM_DIAG11) / 4, Q = (arm - mlx::ARM_DIAG11) % 4;
-                    if (op.kind != mlx::BOP_DIAG || P >= Q) { printf("DIAG11 arm mismatch\n"); exit(1); }
+                    if (op.kind != mlx::BOP_DIAG || P >= Q || Q >= RB) { printf("DIAG11 arm mismatch\n"); exit(1); }
                     for (int j = 0; j < NV; j++) if ((j & (1 << P)) && (j & (1 << Q))) w[j] *= cd(op.m[6], op.m[7]);
                 } else if (arm == mlx::ARM_DIAG) {
                     if (op.kind != mlx::BOP_DIAG) { printf("DIAG arm on a non-diagonal op\n"); exit(1); }
@@ -222,7 +222,8 @@ int main() {
             if (tb > n) continue;
             std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
             std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
-            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1), (trial & 1) ? 8 : 16);
+            const int rb = 3 + ((trial >> 1) & 1);   // both register widths with both amplitude widths
+            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1), (trial & 1) ? 8 : 16, rb);
             std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
             for (const mlx::BPass &P : passes) {
                 if (P.nbits != tb) { printf("register pass with %d tile bits\n", P.nbits); return 1; }
@@ -232,8 +233,8 @@ int main() {
                     std::vector<cd> t(T);
                     auto addr = [&](int l) { uint64_t a = base; for (int b = 0; b < P.nbits; b++) if (l & (1 << b)) a |= 1ull << P.qubits[b]; return a; };
                     for (int l = 0; l < T; l++) t[l] = st[addr(l)];
-                    if (trial & 1) reg_tile<8>(t, P.nbits, &ops[P.first_op], P.num_ops, base, relayouts);
-                    else reg_tile<16>(t, P.nbits, &ops[P.first_op], P.num_ops, base, relayouts);
+                    if (trial & 1) reg_tile<8>(t, P.nbits, rb, &ops[P.first_op], P.num_ops, base, relayouts);
+                    else reg_tile<16>(t, P.nbits, rb, &ops[P.first_op], P.num_ops, base, relayouts);
                     for (int l = 0; l < T; l++) st[addr(l)] = t[l];
                 }
             }

@@ -473,10 +473,12 @@ def test_cache_blocked_application_vs_oracle(eng, oracle):
     assert r.outcomes.tolist() == oracle.job_shots(n, want, 300, 1, 77).tolist()
 
 
-@pytest.mark.parametrize("engine,tile_bits", [("register", 12), ("register", 13), ("lds", 11), ("lds", 12)])
+@pytest.mark.parametrize("engine,tile_bits", [("register", 12), ("register", 13), ("register16", 12),
+                                               ("register8", 13), ("lds", 11), ("lds", 12)])
 def test_blocked_engines_vs_oracle(eng, oracle, engine, tile_bits):
-    """Each blocked engine and tile (register phases on 12- and 13-qubit
-    tiles, LDS ops on 11 and 12) over the whole gate table: complex128 within
+    """Each blocked engine and tile (register phases of 8 or 16 values per
+    thread on 12- and 13-qubit tiles, LDS ops on 11 and 12) over the whole
+    gate table: complex128 within
     1e-12 of the oracle, complex64 by fidelity; 24 qubits gives every
     workgroup several tiles."""
     rng = np.random.default_rng(4100 + tile_bits)
