@@ -199,6 +199,7 @@ typedef struct {
     int                       num_shots;     /* draws: moonlab_qgtl_execute's xorshift64 stream */
     uint64_t                  rng_seed;      /* 0 = clock-based seed, as the reference */
     int                       want_expect_z; /* 1: <Z_q> for every qubit and the norm */
+    int                       want_probabilities; /* 1: the 2^n probabilities (single-state path) */
 } moonlab_mi355x_request_t;
 
 typedef struct {
@@ -212,6 +213,7 @@ typedef struct {
     double    start_time;
     double    finish_time;
     int       path;            /* 0 = co-batched, 1 = single-state, 2 = sequential */
+    double   *probabilities;   /* malloc'd, 2^num_qubits, or NULL */
 } moonlab_mi355x_response_t;
 
 typedef struct {
@@ -231,6 +233,39 @@ MOONLAB_API int moonlab_mi355x_server_wait(moonlab_mi355x_server_t *s, uint64_t 
                                            moonlab_mi355x_response_t *out, double timeout_s);
 MOONLAB_API void moonlab_mi355x_response_free(moonlab_mi355x_response_t *r);
 MOONLAB_API int moonlab_mi355x_server_stats(const moonlab_mi355x_server_t *s, moonlab_mi355x_server_stats_t *st);
+
+/* ---- The control plane's wire format as the serving front door ------------
+ *
+ * Status codes of src/control/control_plane.h:53-62. */
+#ifndef MOONLAB_CONTROL_OK
+#define MOONLAB_CONTROL_OK            ( 0)
+#define MOONLAB_CONTROL_BAD_ARG       (-401)
+#define MOONLAB_CONTROL_OOM           (-402)
+#define MOONLAB_CONTROL_IO_ERROR      (-403)
+#define MOONLAB_CONTROL_PROTOCOL      (-404)
+#define MOONLAB_CONTROL_REJECTED      (-405)
+#endif
+
+/* Addition: one request frame of the control plane's wire protocol
+ * (control_plane.h:9-33) through the serving loop.  Request frames:
+ *   "CIRCUIT <bytes>\n" + moonlab-circuit v1 text  -> "OK <2^n>\n" + 2^n
+ *                                                     little-endian doubles
+ *   "SHOTS <shots> <bytes>\n" + the same text      -> "SAMPLES <shots>\n" +
+ *                                                     shots uint64 outcomes
+ *   "HEALTH\n"                                      -> "OK alive\n"
+ * and "ERR <status> <message>\n" on failure, with the reference's limits
+ * (4 MB body, 2^20 shots, MOONLAB_CONTROL_MAX_QUBITS qubits, default 24,
+ * at most 30) and codes (control_plane.c:294-748).  The circuit runs in
+ * complex128 through the server: concurrent calls, e.g. one per connection
+ * thread of the reference's transport, are co-batched.  seed 0 draws the
+ * shots from a clock-based seed as the control plane does; any other value
+ * seeds the circuit API's xorshift64 stream, so the outcomes equal
+ * moonlab_qgtl_execute's for that seed.  *out_response (malloc'd, free with
+ * free()) holds the response frame to send, error frames included; returns
+ * the MOONLAB_CONTROL_* status.  srv may be NULL for HEALTH and frames the
+ * parser rejects. */
+MOONLAB_API int moonlab_mi355x_server_handle_frame(moonlab_mi355x_server_t *srv, const void *frame, size_t len,
+                                                   uint64_t seed, void **out_response, size_t *out_len);
 
 #ifdef __cplusplus
 }

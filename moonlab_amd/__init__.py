@@ -13,7 +13,7 @@ import ctypes
 import enum
 import os
 from dataclasses import dataclass
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -183,7 +183,7 @@ class _Request(ctypes.Structure):
     _fields_ = [("num_qubits", ctypes.c_int), ("precision", ctypes.c_int),
                 ("gates", ctypes.c_void_p), ("num_gates", ctypes.c_int),
                 ("num_shots", ctypes.c_int), ("rng_seed", ctypes.c_uint64),
-                ("want_expect_z", ctypes.c_int)]
+                ("want_expect_z", ctypes.c_int), ("want_probabilities", ctypes.c_int)]
 
 
 class _Response(ctypes.Structure):
@@ -191,7 +191,8 @@ class _Response(ctypes.Structure):
                 ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
                 ("expect_z", ctypes.POINTER(ctypes.c_double)), ("norm", ctypes.c_double),
                 ("submit_time", ctypes.c_double), ("start_time", ctypes.c_double),
-                ("finish_time", ctypes.c_double), ("path", ctypes.c_int)]
+                ("finish_time", ctypes.c_double), ("path", ctypes.c_int),
+                ("probabilities", ctypes.POINTER(ctypes.c_double))]
 
 
 class _ServerStats(ctypes.Structure):
@@ -269,6 +270,9 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_mi355x_server_wait": ([vp, ctypes.c_uint64, ctypes.POINTER(_Response), ctypes.c_double], ip),
         "moonlab_mi355x_response_free": ([ctypes.POINTER(_Response)], None),
         "moonlab_mi355x_server_stats": ([vp, ctypes.POINTER(_ServerStats)], ip),
+        "moonlab_mi355x_server_handle_frame": ([vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint64,
+                                                ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t)],
+                                               ip),
         "moonlab_mi355x_hea_tfim_energies": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
         "moonlab_mi355x_hea_tfim_gradients": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, dp,
                                                ctypes.POINTER(_BatchStats)], ip),
@@ -999,6 +1003,31 @@ class Response:
     start_time: float
     finish_time: float
     path: int
+    probabilities: Optional[np.ndarray] = None
+
+
+# Status codes of the control plane's wire protocol (control_plane.h:53-62).
+CONTROL_OK, CONTROL_BAD_ARG, CONTROL_OOM, CONTROL_IO_ERROR, CONTROL_PROTOCOL, CONTROL_REJECTED = \
+    0, -401, -402, -403, -404, -405
+
+_libc = ctypes.CDLL(None)
+_libc.free.argtypes = [ctypes.c_void_p]
+_libc.free.restype = None
+
+
+def handle_frame(frame: bytes, server: Optional["Server"] = None, seed: int = 0) -> Tuple[int, bytes]:
+    """One control-plane request frame (CIRCUIT / SHOTS / HEALTH) through
+    moonlab_mi355x_server_handle_frame: returns (status, response frame).
+    Without a server only HEALTH and frames the parser rejects are answered."""
+    out = ctypes.c_void_p()
+    n = ctypes.c_size_t(0)
+    rc = lib().moonlab_mi355x_server_handle_frame(server._h if server is not None else None, frame, len(frame),
+                                                  seed, ctypes.byref(out), ctypes.byref(n))
+    try:
+        data = ctypes.string_at(out.value, n.value) if out.value else b""
+    finally:
+        _libc.free(out)
+    return rc, data
 
 
 class Server:
@@ -1034,9 +1063,10 @@ class Server:
         return lib().moonlab_mi355x_server_now(self._h)
 
     def submit(self, n: int, gates, shots: int = 1024, seed: int = 1, precision: int = C128,
-               expect_z: bool = True) -> int:
+               expect_z: bool = True, probabilities: bool = False) -> int:
         g = gate_array(gates)
-        req = _Request(n, precision, g.ctypes.data if len(g) else None, len(g), shots, seed, int(expect_z))
+        req = _Request(n, precision, g.ctypes.data if len(g) else None, len(g), shots, seed, int(expect_z),
+                       int(probabilities))
         t = ctypes.c_uint64(0)
         _check_sched(lib().moonlab_mi355x_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)), "submit")
         return t.value
@@ -1049,7 +1079,8 @@ class Server:
                 raise SchedulerError(f"server_wait: status {rc}")
             outc = np.ctypeslib.as_array(r.outcomes, (r.num_shots,)).copy() if r.num_shots else None
             z = np.ctypeslib.as_array(r.expect_z, (r.num_qubits,)).copy() if r.expect_z else None
-            return Response(r.status, outc, z, r.norm, r.submit_time, r.start_time, r.finish_time, r.path)
+            p = np.ctypeslib.as_array(r.probabilities, (1 << r.num_qubits,)).copy() if r.probabilities else None
+            return Response(r.status, outc, z, r.norm, r.submit_time, r.start_time, r.finish_time, r.path, p)
         finally:
             lib().moonlab_mi355x_response_free(ctypes.byref(r))
 

@@ -35,7 +35,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libmoonlab_mi355x.so")
 PLUGIN = os.path.join(LIBDIR, "libmoonlab_mi355x_plugin.so")
 # translation units that implement the job / circuit API (left out of the plugin)
-HOST_API = ("scheduler.cpp", "qgtl_circuit.cpp")
+HOST_API = ("scheduler.cpp", "qgtl_circuit.cpp", "control_frame.cpp")
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -288,3 +288,9 @@ moonlab_qgtl_circuit_t *moonlab_qgtl_circuit_load(const char *path, int *out_sta
 }
 
 }  // extern "C"
+
+namespace mlx {
+
+const std::vector<GateRec> &qgtl_circuit_gates(const moonlab_qgtl_circuit_t *c) { return c->gates; }
+
+}  // namespace mlx

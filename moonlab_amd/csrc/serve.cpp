@@ -44,7 +44,7 @@ namespace {
 struct Req {
     uint64_t                        id = 0;
     int                             n = 0, prec = 0, shots = 0;
-    bool                            want_z = false;
+    bool                            want_z = false, want_probs = false;
     std::vector<moonlab_hip_gate_t> gates;
     std::vector<double>             draws;
     uint64_t                        footprint = 0;
@@ -53,6 +53,7 @@ struct Req {
     bool                            done = false;
     std::vector<uint64_t>           outcomes;
     std::vector<double>             z;
+    std::vector<double>             probs;
     double                          norm = 0;
 };
 
@@ -164,6 +165,7 @@ void run_single(Req *r, int mode, hipStream_t stream)
     rq.draws = r->draws.empty() ? nullptr : r->draws.data();
     rq.num_draws = r->shots;
     rq.want_z = r->want_z;
+    rq.want_probs = r->want_probs;
     rq.blocked = mode == 1;
     rq.fusion_max = mode == 2 ? 5 : 0;
     rq.stream = stream;
@@ -172,6 +174,7 @@ void run_single(Req *r, int mode, hipStream_t stream)
     if (r->status == 0) {
         r->outcomes = std::move(rr.outcomes);
         r->z = std::move(rr.z);
+        r->probs = std::move(rr.probs);
         r->norm = rr.norm;
     } else {
         r->status = MOONLAB_SCHED_INTERNAL;
@@ -309,6 +312,7 @@ int moonlab_mi355x_server_submit(moonlab_mi355x_server_t *s, const moonlab_mi355
     r->prec = q->precision;
     r->shots = q->num_shots;
     r->want_z = q->want_expect_z != 0;
+    r->want_probs = q->want_probabilities != 0;
     r->gates.assign(q->gates, q->gates + q->num_gates);
     uint64_t seed = q->rng_seed ? q->rng_seed : ((uint64_t)time(NULL) ^ 0x9e3779b97f4a7c15ULL);
     r->draws.resize((size_t)q->num_shots);
@@ -324,8 +328,10 @@ int moonlab_mi355x_server_submit(moonlab_mi355x_server_t *s, const moonlab_mi355
         r->t_submit = clock_s() - s->t0;
         s->reqs[r->id] = r;
         s->stats.submitted++;
+        // the batch kernel returns observables and shots, not 2^n
+        // probabilities: a probability request takes the single-state path
         if (s->cfg.sequential) s->seq_q.push_back(r);
-        else if (r->n <= s->cfg.batch_max_qubits) s->small_q.push_back(r);
+        else if (r->n <= s->cfg.batch_max_qubits && !r->want_probs) s->small_q.push_back(r);
         else s->large_q.push_back(r);
         *ticket = r->id;
     }
@@ -367,7 +373,12 @@ int moonlab_mi355x_server_wait(moonlab_mi355x_server_t *s, uint64_t ticket, moon
             out->expect_z = (double *)malloc((size_t)r->n * sizeof(double));
             if (out->expect_z) memcpy(out->expect_z, r->z.data(), (size_t)r->n * sizeof(double));
         }
-        if ((r->shots && !out->outcomes) || (r->want_z && !out->expect_z)) out->status = MOONLAB_SCHED_OOM;
+        if (r->want_probs) {
+            out->probabilities = (double *)malloc(r->probs.size() * sizeof(double));
+            if (out->probabilities) memcpy(out->probabilities, r->probs.data(), r->probs.size() * sizeof(double));
+        }
+        if ((r->shots && !out->outcomes) || (r->want_z && !out->expect_z) || (r->want_probs && !out->probabilities))
+            out->status = MOONLAB_SCHED_OOM;
     }
     const int st = out->status;
     delete r;
@@ -379,8 +390,10 @@ void moonlab_mi355x_response_free(moonlab_mi355x_response_t *r)
     if (!r) return;
     free(r->outcomes);
     free(r->expect_z);
+    free(r->probabilities);
     r->outcomes = nullptr;
     r->expect_z = nullptr;
+    r->probabilities = nullptr;
 }
 
 int moonlab_mi355x_server_stats(const moonlab_mi355x_server_t *s, moonlab_mi355x_server_stats_t *st)

@@ -291,3 +291,30 @@ def test_no_device_fails_loudly():
         m.StateVector(4)
     rc, _ = m.Job(2).add_gate(4, 0).set_num_shots(4).run_raw()
     assert rc == m.MOONLAB_SCHED_INTERNAL
+
+
+def test_control_frame_grammar_without_device():
+    """Frames the control plane answers before any execution
+    (control_plane.c:418-425, 590-684): HEALTH, verb and limit errors,
+    deserialize and qubit-cap rejections, with the reference's codes."""
+    hf = m.handle_frame
+    assert hf(b"HEALTH\n") == (m.CONTROL_OK, b"OK alive\n")
+    text = m.Circuit(2).add_gate(m.GateType.H, 0).serialize().encode()
+    cases = [
+        (b"HELLO\n", m.CONTROL_PROTOCOL, b"ERR -404 unknown verb\n"),
+        (b"CIRCUIT x\n", m.CONTROL_PROTOCOL, b"ERR -404 expected CIRCUIT <N>\n"),
+        (b"SHOTS 5\n", m.CONTROL_PROTOCOL, b"ERR -404 expected SHOTS <shots> <bytes>\n"),
+        (b"SHOTS 0 10\n", m.CONTROL_BAD_ARG, b"ERR -401 shots out of range\n"),
+        (b"SHOTS %d 10\n" % ((1 << 20) + 1), m.CONTROL_BAD_ARG, b"ERR -401 shots out of range\n"),
+        (b"CIRCUIT 0\n", m.CONTROL_BAD_ARG, b"ERR -401 body bytes out of range\n"),
+        (b"CIRCUIT %d\n" % ((1 << 22) + 1), m.CONTROL_BAD_ARG, b"ERR -401 body bytes out of range\n"),
+        (b"CIRCUIT 100\n" + text, m.CONTROL_PROTOCOL, b"ERR -404 short body\n"),
+        (b"CIRCUIT 7\nFOO 0\n\n", m.CONTROL_REJECTED, b"ERR -405 deserialize\n"),
+        (b"HEALTH", m.CONTROL_PROTOCOL, b"ERR -404 no verb line\n"),
+    ]
+    for frame, rc, resp in cases:
+        assert hf(frame) == (rc, resp), frame
+    big = b"NUM_QUBITS 25\nH 0\n"
+    assert hf(b"CIRCUIT %d\n" % len(big) + big) == (m.CONTROL_BAD_ARG, b"ERR -401 qubit count exceeds server cap\n")
+    # a valid frame with no server to run it is refused at execution
+    assert hf(b"CIRCUIT %d\n" % len(text) + text) == (m.CONTROL_REJECTED, b"ERR -405 execute\n")

@@ -88,3 +88,67 @@ def test_sequential_baseline_agrees(eng):
         assert b.path == 2
         assert np.max(np.abs(a.expect_z - b.expect_z)) < 1e-12
         assert a.outcomes.tolist() == b.outcomes.tolist()
+
+
+def _frame_circuits(k, seed):
+    """Random circuits over the circuit API's gate set, as (n, gate tuples, text)."""
+    from moonlab_amd import Circuit
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        n = int(rng.integers(3, 15))
+        c, gl = Circuit(n), []
+        for _ in range(int(rng.integers(5, 40))):
+            t = int(rng.integers(0, 14))
+            tg = int(rng.integers(0, n))
+            ctl = int((tg + rng.integers(1, n)) % n) if t >= 10 else -1
+            th = float(rng.uniform(-np.pi, np.pi)) if t in (7, 8, 9) else None
+            c.add_gate(t, tg, ctl, th)
+            gl.append((t, tg, ctl, -1, th if th is not None else 0.0))
+        out.append((n, gl, c.serialize().encode()))
+    return out
+
+
+def _decode(resp, dtype):
+    head, _, body = resp.partition(b"\n")
+    verb, count = head.split(b" ")
+    return verb, np.frombuffer(body, dtype=dtype).copy(), int(count)
+
+
+def test_control_frames_match_reference_vectors(eng, oracle):
+    """CIRCUIT and SHOTS frames through the server against the circuit API's
+    golden vectors (moonlab_qgtl_execute, seed fixed): probabilities within
+    1e-13 (the blocked path folds gate runs into FP64 2x2s), outcomes exact;
+    then 48 frames from 8 threads at once, each identical to its run alone and
+    to the oracle."""
+    import json
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    q = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "shots_ref.json")))["qgtl_rnd10"]
+    text = q["text"].encode()
+    with eng.Server() as srv:
+        assert eng.handle_frame(b"HEALTH\n", srv) == (eng.CONTROL_OK, b"OK alive\n")
+        rc, resp = eng.handle_frame(b"CIRCUIT %d\n" % len(text) + text, srv)
+        verb, p, cnt = _decode(resp, np.float64)
+        assert rc == eng.CONTROL_OK and verb == b"OK" and cnt == 1 << 10 == len(p)
+        assert np.max(np.abs(p - np.array(q["probabilities"]))) < 1e-13
+        rc, resp = eng.handle_frame(b"SHOTS %d %d\n" % (q["shots"], len(text)) + text, srv, seed=q["seed"])
+        verb, o, cnt = _decode(resp, np.uint64)
+        assert rc == eng.CONTROL_OK and verb == b"SAMPLES" and cnt == q["shots"]
+        assert o.tolist() == q["outcomes"]
+
+        circ = _frame_circuits(24, 11)
+        frames = []
+        for i, (n, gl, t) in enumerate(circ):
+            frames.append((b"CIRCUIT %d\n" % len(t) + t, 0))
+            frames.append((b"SHOTS 300 %d\n" % len(t) + t, 1000 + i))
+        alone = [eng.handle_frame(f, srv, seed=s) for f, s in frames]
+        with ThreadPoolExecutor(8) as ex:
+            mixed = list(ex.map(lambda fs: eng.handle_frame(fs[0], srv, seed=fs[1]), frames))
+    assert mixed == alone
+    for i, (n, gl, _) in enumerate(circ):
+        a = oracle.run(n, gl)
+        _, p, _ = _decode(alone[2 * i][1], np.float64)
+        assert np.max(np.abs(p - oracle.probabilities(n, a))) < 1e-13
+        _, o, _ = _decode(alone[2 * i + 1][1], np.uint64)
+        assert o.tolist() == oracle.qgtl_shots(n, a, 300, 1000 + i).tolist()
