@@ -209,11 +209,24 @@ int main() {
     // register-phase programs: 12..15 qubits, tiles of 12 and 13, both
     // amplitude widths of the LDS swizzle
     int reg_checked = 0, relayouts = 0;
-    for (int trial = 0; trial < 16; trial++) {
+    // trials 16..31 add CNOTs and Toffolis controlled from the top qubits,
+    // which are mostly outside the tile set (exchanges on every register
+    // value, on the tiles where the controls hold)
+    for (int trial = 0; trial < 32; trial++) {
         int n = 12 + trial % 4;
         std::vector<moonlab_hip_gate_t> g;
+        // a random product state first, so that every amplitude is nonzero
+        // and no misplaced amplitude goes unseen
+        for (int q = 0; q < n; q++)
+            g.push_back({17, q, -1, -1, (double)(rng() % 1000) / 77.0, (double)(rng() % 1000) / 91.0, (double)(rng() % 1000) / 53.0});
         for (int k = 0; k < 120; k++) {
             int t = 1 + rng() % 24; int q[3]; for (int i = 0; i < 3; i++) { do { q[i] = rng() % n; } while ((i > 0 && q[i] == q[0]) || (i > 1 && q[i] == q[1])); }
+            if (trial >= 16 && rng() % 2 && n >= 13) {
+                t = rng() % 3 ? 10 : 23;
+                q[1] = n - 1 - (int)(rng() % 2);
+                q[2] = q[1] == n - 1 ? n - 2 : n - 1;
+                do { q[0] = rng() % n; } while (q[0] == q[1] || q[0] == q[2]);
+            }
             g.push_back({t, q[0], q[1], q[2], (double)(rng() % 1000) / 77.0, (double)(rng() % 1000) / 91.0, (double)(rng() % 1000) / 53.0});
         }
         std::vector<double> want(2ull << n, 0.0); want[0] = 1.0;
