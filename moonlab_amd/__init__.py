@@ -170,6 +170,10 @@ class _Hea(ctypes.Structure):
                 ("angles", ctypes.POINTER(ctypes.c_double)), ("h", ctypes.c_double)]
 
 
+_HEA_DTYPE = np.dtype([("num_qubits", np.int32), ("num_layers", np.int32), ("angles", np.uintp),
+                       ("h", np.float64)])
+assert _HEA_DTYPE.itemsize == ctypes.sizeof(_Hea)
+
 OBS_Z, OBS_ZZ, OBS_X = 1, 2, 4
 
 
@@ -938,17 +942,23 @@ class Batch:
 
     @staticmethod
     def _hea_array(instances):
-        arr = (_Hea * len(instances))()
-        keep = []
-        for i, (n, L, ang, h) in enumerate(instances):
-            a = np.ascontiguousarray(ang, dtype=np.float64).reshape(-1)
-            assert a.size == L * n * 2
-            keep.append(a)
-            arr[i].num_qubits = n
-            arr[i].num_layers = L
-            arr[i].angles = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-            arr[i].h = h
-        return arr, keep
+        """The moonlab_mi355x_hea_t array, built column-wise: one contiguous
+        angle buffer and one structured array (the layout of _Hea)."""
+        k = len(instances)
+        flats = [np.asarray(x[2], dtype=np.float64).reshape(-1) for x in instances]
+        ns = np.fromiter((x[0] for x in instances), np.int32, k)
+        ls = np.fromiter((x[1] for x in instances), np.int32, k)
+        sizes = 2 * ns.astype(np.int64) * ls
+        got = np.fromiter((f.size for f in flats), np.int64, k)
+        if np.any(got != sizes):
+            raise ValueError("angles must hold layers x qubits x 2 values per instance")
+        flat = np.concatenate(flats) if k else np.zeros(1)
+        arr = np.zeros(max(k, 1), _HEA_DTYPE)
+        arr["num_qubits"][:k] = ns
+        arr["num_layers"][:k] = ls
+        arr["angles"][:k] = flat.ctypes.data + 8 * (np.cumsum(sizes) - sizes)
+        arr["h"][:k] = np.fromiter((x[3] for x in instances), np.float64, k)
+        return arr.ctypes.data_as(ctypes.POINTER(_Hea)), (arr, flat)
 
     def hea_tfim_gradients(self, instances, precision: int = C128):
         """Energies and parameter-shift gradients ([layers, n, 2] per instance),
@@ -971,16 +981,7 @@ class Batch:
 
     def hea_tfim(self, instances, precision: int = C128) -> np.ndarray:
         """instances: (n, layers, angles[layers, n, 2], h) tuples -> energies."""
-        arr = (_Hea * len(instances))()
-        keep = []
-        for i, (n, L, ang, h) in enumerate(instances):
-            a = np.ascontiguousarray(ang, dtype=np.float64).reshape(-1)
-            assert a.size == L * n * 2
-            keep.append(a)
-            arr[i].num_qubits = n
-            arr[i].num_layers = L
-            arr[i].angles = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-            arr[i].h = h
+        arr, keep = self._hea_array(instances)
         e = np.zeros(len(instances))
         st = _BatchStats()
         _check_sched(lib().moonlab_mi355x_hea_tfim_energies(

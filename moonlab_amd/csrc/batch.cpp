@@ -568,6 +568,29 @@ int grow_host(void **p, size_t *cap, size_t need)
     return 0;
 }
 
+// fn(t) for t = 0..nthreads-1 on host threads (t = 0 and any that could not
+// be started on the calling thread)
+template <typename F>
+void on_threads(int nthreads, F &&fn)
+{
+    std::vector<std::thread> th;
+    int started = 0;
+    try {
+        for (int t = 1; t < nthreads; t++, started++) th.emplace_back(fn, t);
+    } catch (const std::system_error &) {
+        // no more host threads: the calling thread runs what was not handed out
+    }
+    fn(0);
+    for (int t = 1 + started; t < nthreads; t++) fn(t);
+    for (std::thread &x : th) x.join();
+}
+
+int host_threads(int work_items)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    return work_items < 64 ? 1 : (int)std::min<unsigned>(16u, hw ? hw : 1u);
+}
+
 int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int count,
               moonlab_mi355x_batch_stats_t *stats)
 {
@@ -576,8 +599,6 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     if (count == 0) return MOONLAB_SCHED_OK;
     const int tbmax = b->tile_bits;
     std::vector<BInst> insts((size_t)count);
-    std::vector<BPass> passes;
-    std::vector<BOp> ops;
     std::vector<double> draws;
     std::vector<std::pair<double, int>> cost((size_t)count);
     int64_t obs_total = 0, out_total = 0;
@@ -592,8 +613,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         std::vector<BOp>   ops;
         bool               bad = false;
     };
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int nthreads = count < 64 ? 1 : (int)std::min<unsigned>(16u, hw ? hw : 1u);
+    const int nthreads = host_threads(count);
     std::vector<Chunk> chunks((size_t)nthreads);
     auto plan_chunk = [&](Chunk &c) {
         std::vector<GOp> gops;
@@ -615,7 +635,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
             I.n = n;
             I.prec = it.precision;
             I.tile_bits = n < tbmax ? n : tbmax;
-            I.first_pass = (int)c.passes.size();   // chunk-local (passes and ops) until concatenation
+            I.first_pass = (int)c.passes.size();   // chunk-local (passes and ops) until placed
             plan_passes(gops, n, I.tile_bits, kSegRegsBatch, c.passes, c.ops);
             I.num_passes = (int)c.passes.size() - I.first_pass;
             // longest first: sweeps x amplitudes (resident circuits: ops x tile)
@@ -627,34 +647,19 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         chunks[(size_t)t].lo = (int)((int64_t)count * t / nthreads);
         chunks[(size_t)t].hi = (int)((int64_t)count * (t + 1) / nthreads);
     }
-    {
-        std::vector<std::thread> th;
-        int started = 0;
-        try {
-            for (int t = 1; t < nthreads; t++, started++) th.emplace_back(plan_chunk, std::ref(chunks[(size_t)t]));
-        } catch (const std::system_error &) {
-            // no more host threads: the calling thread plans what was not handed out
-        }
-        plan_chunk(chunks[0]);
-        for (int t = 1 + started; t < nthreads; t++) plan_chunk(chunks[(size_t)t]);
-        for (std::thread &x : th) x.join();
-    }
+    on_threads(nthreads, [&](int t) { plan_chunk(chunks[(size_t)t]); });
+    // each chunk's passes and ops land at their offsets in the program arena
+    // (copied below by the planning threads, straight into pinned memory)
+    std::vector<size_t> pass_base((size_t)nthreads), op_base((size_t)nthreads);
     size_t total_passes = 0, total_ops = 0;
-    for (const Chunk &c : chunks) {
+    for (int t = 0; t < nthreads; t++) {
+        const Chunk &c = chunks[(size_t)t];
         if (c.bad) return MOONLAB_SCHED_BAD_ARG;
+        pass_base[(size_t)t] = total_passes;
+        op_base[(size_t)t] = total_ops;
         total_passes += c.passes.size();
         total_ops += c.ops.size();
-    }
-    passes.reserve(total_passes);
-    ops.reserve(total_ops);
-    for (const Chunk &c : chunks) {
-        const int pass_base = (int)passes.size(), op_base = (int)ops.size();
-        for (BPass P : c.passes) {
-            P.first_op += op_base;
-            passes.push_back(P);
-        }
-        ops.insert(ops.end(), c.ops.begin(), c.ops.end());
-        for (int i = c.lo; i < c.hi; i++) insts[(size_t)i].first_pass += pass_base;
+        for (int i = c.lo; i < c.hi; i++) insts[(size_t)i].first_pass += (int)pass_base[(size_t)t];
     }
     // observables, draws and outcome slots in circuit order
     for (int i = 0; i < count; i++) {
@@ -701,8 +706,8 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     auto place = [&](size_t bytes) { const size_t o = off; off = (off + bytes + 255) / 256 * 256; return o; };
     const size_t o_inst = place(insts.size() * sizeof(BInst));
     const size_t o_order = place(order.size() * sizeof(int32_t));
-    const size_t o_pass = place((passes.size() + 1) * sizeof(BPass));
-    const size_t o_ops = place((ops.size() + 1) * sizeof(BOp));
+    const size_t o_pass = place((total_passes + 1) * sizeof(BPass));
+    const size_t o_ops = place((total_ops + 1) * sizeof(BOp));
     const size_t o_draw = place((draws.size() + 1) * sizeof(double));
     const size_t prog_bytes = off;
     off = 0;
@@ -715,8 +720,20 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     char *hp = (char *)b->h_prog;
     memcpy(hp + o_inst, insts.data(), insts.size() * sizeof(BInst));
     memcpy(hp + o_order, order.data(), order.size() * sizeof(int32_t));
-    if (!passes.empty()) memcpy(hp + o_pass, passes.data(), passes.size() * sizeof(BPass));
-    if (!ops.empty()) memcpy(hp + o_ops, ops.data(), ops.size() * sizeof(BOp));
+    {
+        BPass *hpass = (BPass *)(hp + o_pass);
+        BOp *hops = (BOp *)(hp + o_ops);
+        auto place_chunk = [&](int t) {
+            const Chunk &c = chunks[(size_t)t];
+            BPass *dst = hpass + pass_base[(size_t)t];
+            for (size_t k = 0; k < c.passes.size(); k++) {
+                dst[k] = c.passes[k];
+                dst[k].first_op += (int32_t)op_base[(size_t)t];
+            }
+            if (!c.ops.empty()) memcpy(hops + op_base[(size_t)t], c.ops.data(), c.ops.size() * sizeof(BOp));
+        };
+        on_threads(nthreads, place_chunk);
+    }
     if (!draws.empty()) memcpy(hp + o_draw, draws.data(), draws.size() * sizeof(double));
     char *dp = (char *)b->d_prog;
     char *dr = (char *)b->d_res;
@@ -810,7 +827,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         for (const BInst &I : insts)
             if (I.n > I.tile_bits) big_passes += I.num_passes;
         stats->total_passes = big_passes;
-        stats->total_ops = (int)ops.size();
+        stats->total_ops = (int)total_ops;
         stats->graph_replayed = replayed ? 1 : 0;
         stats->reserved = 0;
     }
@@ -989,32 +1006,44 @@ int moonlab_mi355x_hea_tfim_energies(moonlab_mi355x_batch_t *b, const moonlab_mi
     }
     zz.assign(zoff, 0.0);
     xx.assign(xoff, 0.0);
+    std::vector<size_t> zo((size_t)count), xo((size_t)count);
     zoff = xoff = 0;
     for (int i = 0; i < count; i++) {
         const int n = inst[i].num_qubits, L = inst[i].num_layers;
         if (n < 1 || n > kBatchMaxQubits || L < 0 || (L > 0 && !inst[i].angles)) return MOONLAB_SCHED_BAD_ARG;
-        auto &g = gl[(size_t)i];
-        for (int l = 0; l < L; l++) {
-            for (int q = 0; q < n; q++) {
-                const double *a = inst[i].angles + ((size_t)l * n + q) * 2;
-                g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_RY, q, -1, -1, a[0], 0, 0});
-                g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_RZ, q, -1, -1, a[1], 0, 0});
-            }
-            for (int q = 0; q + 1 < n; q++)
-                g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_CZ, q + 1, q, -1, 0, 0, 0});
-        }
-        moonlab_mi355x_batch_item_t &it = items[(size_t)i];
-        memset(&it, 0, sizeof it);
-        it.num_qubits = n;
-        it.precision = precision;
-        it.gates = g.data();
-        it.num_gates = (int)g.size();
-        it.observables = MOONLAB_MI355X_OBS_ZZ | MOONLAB_MI355X_OBS_X;
-        it.zz_chain = zz.data() + zoff;
-        it.expect_x = xx.data() + xoff;
+        zo[(size_t)i] = zoff;
+        xo[(size_t)i] = xoff;
         zoff += (size_t)(n > 1 ? n - 1 : 1);
         xoff += (size_t)n;
     }
+    // the ansatz gate lists (vqe.c:764-781 layer order), built on host threads
+    const int nthreads = host_threads(count);
+    on_threads(nthreads, [&](int t) {
+        const int lo = (int)((int64_t)count * t / nthreads), hi = (int)((int64_t)count * (t + 1) / nthreads);
+        for (int i = lo; i < hi; i++) {
+            const int n = inst[i].num_qubits, L = inst[i].num_layers;
+            auto &g = gl[(size_t)i];
+            g.reserve((size_t)L * (size_t)(3 * n - 1));
+            for (int l = 0; l < L; l++) {
+                for (int q = 0; q < n; q++) {
+                    const double *a = inst[i].angles + ((size_t)l * n + q) * 2;
+                    g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_RY, q, -1, -1, a[0], 0, 0});
+                    g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_RZ, q, -1, -1, a[1], 0, 0});
+                }
+                for (int q = 0; q + 1 < n; q++)
+                    g.push_back(moonlab_hip_gate_t{MOONLAB_HIP_GATE_CZ, q + 1, q, -1, 0, 0, 0});
+            }
+            moonlab_mi355x_batch_item_t &it = items[(size_t)i];
+            memset(&it, 0, sizeof it);
+            it.num_qubits = n;
+            it.precision = precision;
+            it.gates = g.data();
+            it.num_gates = (int)g.size();
+            it.observables = MOONLAB_MI355X_OBS_ZZ | MOONLAB_MI355X_OBS_X;
+            it.zz_chain = zz.data() + zo[(size_t)i];
+            it.expect_x = xx.data() + xo[(size_t)i];
+        }
+    });
     const int rc = run_batch(b, items.data(), count, stats);
     if (rc) return rc;
     // E = -sum_q <Z_q Z_q+1> - h sum_q <X_q>, summed in qubit order
