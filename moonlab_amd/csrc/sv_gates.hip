@@ -35,6 +35,12 @@ namespace {
 
 constexpr int kMaxBlock = 1024;
 
+// linear block index of a 1-D or 2-D grid (grid_for)
+__device__ __forceinline__ uint64_t block_id()
+{
+    return (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+}
+
 // --- pair kernels -----------------------------------------------------------
 
 template <typename R, typename Op, int E, bool NT>
@@ -42,7 +48,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(cplx_t<R> *__restrict__ a, u
                                                     int q, Op op)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
     const uint64_t stride = 1ull << q;
     if (base + (uint64_t)(E - 1) * bs < npairs) {
         uint64_t i0[E];
@@ -78,7 +84,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_half(cplx_t<R> *__restrict__ a, u
                                                     int q, Op op)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
     const uint64_t bit = 1ull << q;
     if (base + (uint64_t)(E - 1) * bs < nhalf) {
         uint64_t i[E];
@@ -111,7 +117,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_diag(cplx_t<R> *__restrict__ a, u
                                                     int q, R d0r, R d0i, R d1r, R d1i)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
     const bool full = base + (uint64_t)(E - 1) * bs < dim;
     cplx_t<R> x[E];
 #pragma unroll
@@ -136,7 +142,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_ctrl_pair(cplx_t<R> *__restrict__
                                                          uint64_t tbit, Op op)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
     if (base + (uint64_t)(E - 1) * bs < nsub) {
         uint64_t i0[E];
         cplx_t<R> x0[E], x1[E];
@@ -171,7 +177,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_mask_elem(cplx_t<R> *__restrict__
                                                          BitSet fixed, uint64_t mask, Op op)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
     if (base + (uint64_t)(E - 1) * bs < nsub) {
         uint64_t i[E];
         cplx_t<R> x[E];
@@ -204,7 +210,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_swap_bits(cplx_t<R> *__restrict__
                                                          uint64_t b1, uint64_t b2)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const uint64_t p = base + (uint64_t)k * bs;
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_mat4(cplx_t<R> *__restrict__ a, u
                                                     Mat4<R> M)
 {
     const uint64_t bs = blockDim.x;
-    const uint64_t base = (uint64_t)blockIdx.x * bs * E + threadIdx.x;
+    const uint64_t base = block_id() * bs * E + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const uint64_t p = base + (uint64_t)k * bs;
@@ -282,10 +288,18 @@ __global__ __launch_bounds__(256) void k_convert(const cplx_t<RS> *__restrict__ 
 
 // --- launch helpers -----------------------------------------------------------
 
-inline unsigned grid_for(uint64_t work, int block, int e)
+// One-shot launch shape for `work` items, e per thread.  A launch holds at
+// most 2^32 - 1 work-items per grid dimension, which 1-qubit gates on 33+
+// qubits (complex128) exceed; such grids go 2-D, 65536 blocks per row, and
+// block_id() linearizes them (blocks past the end fail the kernels' bounds
+// checks).
+inline dim3 grid_for(uint64_t work, int block, int e)
 {
     const uint64_t per = (uint64_t)block * (uint64_t)e;
-    return (unsigned)((work + per - 1) / per);
+    const uint64_t blocks = (work + per - 1) / per;
+    if (blocks * (uint64_t)block < (1ull << 31)) return dim3((unsigned)blocks, 1, 1);
+    constexpr uint64_t kRow = 65536;
+    return dim3((unsigned)kRow, (unsigned)((blocks + kRow - 1) / kRow), 1);
 }
 
 inline unsigned grid_stride_blocks(uint64_t work)

@@ -543,3 +543,83 @@ def test_concurrent_jobs_match_sequential(eng):
     assert mixed == alone * 3
     for a, b in zip(mixed_z, alone_z * 2):
         assert _bytes_equal(a, b)
+
+
+def _inverse(g):
+    """The random-layer circuit's inverse: reversed, rotation angles negated
+    (CNOT is its own inverse)."""
+    out = []
+    for row in g[::-1]:
+        t = int(row["type"])
+        th = -float(row["theta"]) if t in (7, 8, 9) else 0.0
+        out.append((t, int(row["target"]), int(row["control"]), -1, th))
+    return out
+
+
+@pytest.mark.parametrize("n,prec,path,tol", [
+    (30, "c128", "gates", 1e-12),     # the 16 GiB sweep state, one kernel per gate
+    (30, "c128", "blocked", 1e-12),   # register-phase passes
+    (30, "c64", "blocked", 2e-5),
+    (32, "c64", "fused", 2e-5),       # 32 GiB, FP32 MFMA blocks
+    (33, "c128", "fused", 1e-12),     # 128 GiB, FP64 MFMA blocks
+])
+def test_full_size_round_trips(eng, n, prec, path, tol):
+    """At the benchmark sizes, through size-independent properties: the norm
+    stays 1 after a random-layer circuit, and the circuit followed by its
+    inverse returns |0..0> (amplitude 0 within tol of 1, every <Z_q> = 1)."""
+    from moonlab_amd import workloads
+    g = workloads.random_layer_circuit(n, 2, 17, h_layer=True)
+    p = eng.C128 if prec == "c128" else eng.C64
+    kw = {"blocked": True} if path == "blocked" else {"fuse": 5} if path == "fused" else {}
+    print(f"round trip n={n} {prec} {path}", flush=True)
+    if n >= 32:   # room for the state: drop the job paths' parked states
+        eng.lib().moonlab_mi355x_set_state_cache_bytes(0)
+        eng.lib().moonlab_mi355x_set_state_cache_bytes(64 << 30)
+    with eng.StateVector(n, p) as s:
+        s.apply(g, **kw)
+        _, nrm = s.expect_z_all()
+        assert abs(nrm - 1.0) < (1e-10 if prec == "c128" else 1e-5)
+        s.apply(_inverse(g), **kw)
+        head = s.read(0, 2)
+        z, nrm = s.expect_z_all()
+    assert abs(head[0] - 1.0) < tol and abs(head[1]) < tol
+    assert np.max(np.abs(z - 1.0)) < 4 * tol
+
+
+def test_gate_families_at_33_qubits(eng):
+    """Every one-shot gate kernel on the largest complex128 state one MI355X
+    holds (128 GiB; launches of 2^32 work-items and more): a product of one
+    gate of each family followed by its inverse returns |0..0>."""
+    n = 33
+    fwd = [(4, 0), (4, 32), (4, 17), (8, 32, -1, -1, 0.7), (7, 1, -1, -1, 0.4),
+           (9, 32, -1, -1, 1.1),                 # diagonal on every element
+           (5, 32), (6, 2),                      # phase on the |1> half
+           (10, 0, 32), (10, 32, 0),             # controlled pairs
+           (12, 17, 32),                         # CZ: masked elements
+           (13, 0, 32),                          # SWAP
+           (18, 1, 32, -1, 0.3),                 # CPHASE
+           (20, 32, 17, -1, 0.9),                # CRY
+           (23, 5, 32, 0),                       # Toffoli
+           (24, 32, 1, 2),                       # Fredkin
+           (17, 32, -1, -1, 0.3, 0.2, 0.1)]      # U3
+    inv = {5: 14, 6: 15}                          # S -> S^dagger, T -> T^dagger
+    back = []
+    for g in fwd[::-1]:
+        t = g[0]
+        if t in (7, 8, 9, 18, 20):
+            back.append((t, g[1], g[2], g[3], -g[4]))
+        elif t == 17:   # U3(theta, phi, lam)^-1 = U3(-theta, -lam, -phi)
+            back.append((17, g[1], -1, -1, -g[4], -g[6], -g[5]))
+        else:
+            back.append((inv.get(t, t),) + tuple(g[1:]))
+    eng.lib().moonlab_mi355x_set_state_cache_bytes(0)
+    eng.lib().moonlab_mi355x_set_state_cache_bytes(64 << 30)
+    with eng.StateVector(n) as s:
+        s.apply(fwd)
+        _, nrm = s.expect_z_all()
+        assert abs(nrm - 1.0) < 1e-12
+        s.apply(back)
+        head = s.read(0, 2)
+        z, _ = s.expect_z_all()
+    assert abs(head[0] - 1.0) < 1e-12 and abs(head[1]) < 1e-12
+    assert np.max(np.abs(z - 1.0)) < 1e-12
