@@ -586,11 +586,12 @@ def test_full_size_round_trips(eng, n, prec, path, tol):
     assert np.max(np.abs(z - 1.0)) < 4 * tol
 
 
-def test_gate_families_at_33_qubits(eng):
-    """Every one-shot gate kernel on the largest complex128 state one MI355X
-    holds (128 GiB; launches of 2^32 work-items and more): a product of one
-    gate of each family followed by its inverse returns |0..0>."""
-    n = 33
+@pytest.mark.parametrize("n", [33, 34])
+def test_gate_families_at_33_and_34_qubits(eng, n):
+    """Every one-shot gate kernel on complex128 states of 128 and 256 GiB (34
+    qubits is the largest one MI355X holds; launches of 2^32 work-items and
+    more): a product of one gate of each family followed by its inverse
+    returns |0..0>."""
     fwd = [(4, 0), (4, 32), (4, 17), (8, 32, -1, -1, 0.7), (7, 1, -1, -1, 0.4),
            (9, 32, -1, -1, 1.1),                 # diagonal on every element
            (5, 32), (6, 2),                      # phase on the |1> half
