@@ -562,6 +562,7 @@ def _inverse(g):
     (30, "c64", "blocked", 2e-5),
     (32, "c64", "fused", 2e-5),       # 32 GiB, FP32 MFMA blocks
     (33, "c128", "fused", 1e-12),     # 128 GiB, FP64 MFMA blocks
+    (33, "c128", "blocked", 1e-12),
 ])
 def test_full_size_round_trips(eng, n, prec, path, tol):
     """At the benchmark sizes, through size-independent properties: the norm
@@ -582,7 +583,9 @@ def test_full_size_round_trips(eng, n, prec, path, tol):
         s.apply(_inverse(g), **kw)
         head = s.read(0, 2)
         z, nrm = s.expect_z_all()
+        shots = s.sample(np.linspace(0.0, 0.999, 64)) if prec == "c128" else np.zeros(1)
     assert abs(head[0] - 1.0) < tol and abs(head[1]) < tol
+    assert not np.any(shots)   # |0..0>: the sampler's CDF over the whole state puts every draw on 0
     assert np.max(np.abs(z - 1.0)) < 4 * tol
 
 
