@@ -318,3 +318,29 @@ def test_control_frame_grammar_without_device():
     assert hf(b"CIRCUIT %d\n" % len(big) + big) == (m.CONTROL_BAD_ARG, b"ERR -401 qubit count exceeds server cap\n")
     # a valid frame with no server to run it is refused at execution
     assert hf(b"CIRCUIT %d\n" % len(text) + text) == (m.CONTROL_REJECTED, b"ERR -405 execute\n")
+
+
+def test_control_frame_fuzz_without_device():
+    """Mutated and random frames never crash the parser and always get one
+    well-formed reply line (the reference fuzzes its control plane the same
+    way; a frame it would execute is refused here for want of a server)."""
+    rng = np.random.default_rng(20261202)
+    text = m.Circuit(3).add_gate(m.GateType.H, 0).add_gate(m.GateType.CNOT, 1, 0).serialize().encode()
+    seeds = [b"HEALTH\n", b"CIRCUIT %d\n" % len(text) + text, b"SHOTS 16 %d\n" % len(text) + text]
+    codes = {m.CONTROL_OK, m.CONTROL_BAD_ARG, m.CONTROL_PROTOCOL, m.CONTROL_REJECTED}
+    for i in range(3000):
+        base = bytearray(seeds[i % 3])
+        k = int(rng.integers(0, 3))
+        if k == 0:      # flip bytes
+            for _ in range(int(rng.integers(1, 6))):
+                base[int(rng.integers(0, len(base)))] = int(rng.integers(0, 256))
+        elif k == 1:    # truncate
+            base = base[: int(rng.integers(0, len(base) + 1))]
+        else:           # random bytes
+            base = bytearray(rng.integers(0, 256, int(rng.integers(0, 200)), dtype=np.uint8).tobytes())
+        rc, resp = m.handle_frame(bytes(base))
+        assert rc in codes, (bytes(base), rc)
+        if rc == m.CONTROL_OK:
+            assert resp == b"OK alive\n"
+        else:
+            assert resp.startswith(b"ERR %d " % rc) and resp.endswith(b"\n") and resp.count(b"\n") == 1
