@@ -344,3 +344,21 @@ def test_control_frame_fuzz_without_device():
             assert resp == b"OK alive\n"
         else:
             assert resp.startswith(b"ERR %d " % rc) and resp.endswith(b"\n") and resp.count(b"\n") == 1
+
+
+def test_control_frame_parsers_under_sanitizers(tmp_path):
+    """tests/frame_fuzz.cpp: 200k fuzzed frames through the frame and circuit
+    parsers built with AddressSanitizer + UBSan (host code only)."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    exe = str(tmp_path / "frame_fuzz")
+    csrc = os.path.join(ROOT, "moonlab_amd", "csrc")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-I" + os.path.join(ROOT, "include"), "-I" + csrc, os.path.join(ROOT, "tests", "frame_fuzz.cpp"),
+                    os.path.join(csrc, "control_frame.cpp"), os.path.join(csrc, "qgtl_circuit.cpp"), "-o", exe],
+                   check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("fuzzed 200000 frames"), r.stdout
