@@ -244,6 +244,32 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_sample(const moonlab_hip_state_t *s
                                                     const double *u, int num_samples,
                                                     uint64_t *out);
 
+/* ---- Collapsing measurements ------------------------------------------------ */
+
+/* Replaces measurement_single_qubit (src/quantum/measurement.c:181-231): the
+ * outcome is 1 when u < P(1) (the unnormalized sum over the qubit's |1> half),
+ * else 0; the other half is zeroed and the kept half scaled by 1 / sqrt(its
+ * norm), FP64.  When the kept norm is at most 1e-15 (MEASUREMENT_SMALL_NORM)
+ * *outcome is -1 and the state is left zeroed but not rescaled, as the
+ * reference leaves it. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_measure_qubit(moonlab_hip_state_t *state, uint32_t qubit,
+                                                           double u, int *outcome);
+
+/* Replaces measurement_partial (src/quantum/measurement.c:320-410): outcome
+ * bit j is qubit qubits[j]; the 2^k outcome probabilities (FP64, one device
+ * pass), the first outcome whose cumulative probability exceeds u (else the
+ * last outcome of nonzero probability), then collapse and renormalization
+ * (skipped at or below the 1e-15 norm floor).  k in 1..20 distinct qubits
+ * (the reference also accepts repeats and larger k). */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_measure_partial(moonlab_hip_state_t *state, const int *qubits,
+                                                             int k, double u, uint64_t *outcome);
+
+/* Replaces measurement_all_qubits (src/quantum/measurement.c:263-307): the
+ * outcome of moonlab_hip_sample for u, then the state becomes that basis
+ * state with amplitude exactly 1. */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_measure_all(moonlab_hip_state_t *state, double u,
+                                                         uint64_t *outcome);
+
 /* ---- Host transfer (cuda_statevec.h:152-178) ------------------------------- */
 
 /* Replaces moonlab_cuda_state_copy_to_host (cuda_statevec.h:152-160): 2 * dim

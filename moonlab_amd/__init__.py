@@ -320,6 +320,10 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_probabilities": ([vp, dp], ip),
         "moonlab_hip_inner_product": ([vp, vp, dp, dp], ip),
         "moonlab_hip_normalize": ([vp], ip),
+        "moonlab_hip_measure_qubit": ([vp, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_int)], ip),
+        "moonlab_hip_measure_partial": ([vp, ctypes.POINTER(ctypes.c_int), ip, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_uint64)], ip),
+        "moonlab_hip_measure_all": ([vp, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)], ip),
         "moonlab_hip_sample": ([vp, dp, ip, u64p], ip),
         "moonlab_hip_state_copy_to_host": ([vp, dp], ip),
         "moonlab_hip_state_copy_from_host": ([vp, dp], ip),
@@ -771,6 +775,28 @@ class StateVector:
     def normalize(self) -> "StateVector":
         _check_hip(lib().moonlab_hip_normalize(self._h), "normalize")
         return self
+
+    def measure_qubit(self, qubit: int, u: float) -> int:
+        """Collapse one qubit (measurement_single_qubit): 0 or 1, or -1 when
+        the drawn outcome carries no weight."""
+        r = ctypes.c_int(0)
+        _check_hip(lib().moonlab_hip_measure_qubit(self._h, qubit, u, ctypes.byref(r)), "measure_qubit")
+        return r.value
+
+    def measure_partial(self, qubits, u: float) -> int:
+        """Collapse the given qubits (measurement_partial); bit j of the
+        result is qubits[j]."""
+        qs = (ctypes.c_int * len(qubits))(*[int(q) for q in qubits])
+        r = ctypes.c_uint64(0)
+        _check_hip(lib().moonlab_hip_measure_partial(self._h, qs, len(qubits), u, ctypes.byref(r)),
+                   "measure_partial")
+        return r.value
+
+    def measure_all(self, u: float) -> int:
+        """Collapse every qubit (measurement_all_qubits)."""
+        r = ctypes.c_uint64(0)
+        _check_hip(lib().moonlab_hip_measure_all(self._h, u, ctypes.byref(r)), "measure_all")
+        return r.value
 
     def sample(self, u) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64)

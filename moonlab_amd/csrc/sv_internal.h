@@ -65,6 +65,13 @@ struct ReduceWs {
 void       reduce_geometry(int n, int *tile_bits, uint64_t *ntiles, int *blocks);
 hipError_t launch_reduce_z(const SvDev &s, const ReduceWs &w, bool want_tiles);
 hipError_t launch_tile_scan(const ReduceWs &w, hipStream_t st);
+// Collapsing measurements (sv_measure.hip): the FP64 probabilities of the 2^k
+// outcomes of qubits[0..k-1] (outcome bit j = qubits[j], distinct, k <= 20)
+// into host_out (synchronizes the stream); and the collapse pass (mode 0:
+// zero the amplitudes whose masked bits differ from want, 1: also scale the
+// rest by inv, 2: the basis state want with amplitude 1).
+hipError_t launch_outcome_probs(const SvDev &s, const int *qubits, int k, double *host_out);
+hipError_t launch_collapse(const SvDev &s, uint64_t mask, uint64_t want, double inv, int mode);
 hipError_t launch_sample(const SvDev &s, const ReduceWs &w, const double *d_u, int nshots,
                          uint64_t *d_out);
 hipError_t launch_probabilities(const SvDev &s, double *d_p);

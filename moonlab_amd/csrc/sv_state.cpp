@@ -534,6 +534,87 @@ moonlab_hip_status_t moonlab_hip_sample(const moonlab_hip_state_t *cs, const dou
     return drv(e, "sample");
 }
 
+// Collapsing measurements (measurement.c:181-231, 263-307, 320-410): the
+// outcome sums come from one device pass, the outcome rule runs on the host
+// over 2^k doubles as the reference's does, and one more pass collapses.
+// Below MEASUREMENT_SMALL_NORM (1e-15, measurement.c:33) the kept amplitudes
+// are not rescaled.
+moonlab_hip_status_t moonlab_hip_measure_qubit(moonlab_hip_state_t *s, uint32_t qubit, double u, int *outcome)
+{
+    if (!s || !outcome) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");
+    if (qubit >= (uint32_t)s->n) return fail(MOONLAB_HIP_ERR_INVALID_QUBIT, "qubit out of range");
+    const SvDev d = dev(s);
+    const int q = (int)qubit;
+    double p[2] = {0.0, 0.0};
+    hipError_t e = launch_outcome_probs(d, &q, 1, p);
+    if (e != hipSuccess) return drv(e, "measure_qubit");
+    const int r = u < p[1] ? 1 : 0;   // P(1) unnormalized, as measurement_probability_one
+    const uint64_t mask = 1ull << q, want = (uint64_t)r << q;
+    if (p[r] <= 1e-15) {
+        e = launch_collapse(d, mask, want, 1.0, 0);
+        *outcome = -1;
+    } else {
+        e = launch_collapse(d, mask, want, 1.0 / sqrt(p[r]), 1);
+        *outcome = r;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return drv(e, "measure_qubit");
+}
+
+moonlab_hip_status_t moonlab_hip_measure_partial(moonlab_hip_state_t *s, const int *qubits, int k, double u,
+                                                 uint64_t *outcome)
+{
+    if (!s || !qubits || !outcome) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");
+    if (k < 1 || k > 20 || k > s->n) return fail(MOONLAB_HIP_ERR_BAD_ARG, "measure 1..20 qubits");
+    uint64_t mask = 0;
+    for (int j = 0; j < k; j++) {
+        if (qubits[j] < 0 || qubits[j] >= s->n) return fail(MOONLAB_HIP_ERR_INVALID_QUBIT, "qubit out of range");
+        if (mask & (1ull << qubits[j])) return fail(MOONLAB_HIP_ERR_BAD_ARG, "repeated qubit");
+        mask |= 1ull << qubits[j];
+    }
+    const SvDev d = dev(s);
+    std::vector<double> p((size_t)1 << k);
+    hipError_t e = launch_outcome_probs(d, qubits, k, p.data());
+    if (e != hipSuccess) return drv(e, "measure_partial");
+    // the cumulative rule with the last-nonzero fallback (measurement.c:357-376)
+    double cum = 0.0;
+    uint64_t r = 0, last_nz = 0;
+    bool found = false, have_nz = false;
+    for (uint64_t o = 0; o < p.size(); o++) {
+        if (p[o] > 0.0) {
+            last_nz = o;
+            have_nz = true;
+        }
+        cum += p[o];
+        if (u < cum) {
+            r = o;
+            found = true;
+            break;
+        }
+    }
+    if (!found) r = have_nz ? last_nz : 0;
+    uint64_t want = 0;
+    for (int j = 0; j < k; j++)
+        if ((r >> j) & 1ull) want |= 1ull << qubits[j];
+    const double nf = p[r];
+    e = nf > 1e-15 ? launch_collapse(d, mask, want, 1.0 / sqrt(nf), 1) : launch_collapse(d, mask, want, 1.0, 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    *outcome = r;
+    return drv(e, "measure_partial");
+}
+
+moonlab_hip_status_t moonlab_hip_measure_all(moonlab_hip_state_t *s, double u, uint64_t *outcome)
+{
+    if (!s || !outcome) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");
+    uint64_t r = 0;
+    const moonlab_hip_status_t st = moonlab_hip_sample(s, &u, 1, &r);
+    if (st) return st;
+    hipError_t e = launch_collapse(dev(s), s->dim - 1, r, 1.0, 2);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    *outcome = r;
+    return drv(e, "measure_all");
+}
+
 moonlab_hip_status_t moonlab_hip_state_read(const moonlab_hip_state_t *s, uint64_t off, uint64_t count, void *out)
 {
     if (!s || (count && !out)) return fail(MOONLAB_HIP_ERR_BAD_ARG, "NULL argument");

@@ -83,6 +83,13 @@ class Oracle:
         L.svo_fidelity.restype = ctypes.c_double
         L.svo_measure_all_outcome.argtypes = [ctypes.c_int, dp, ctypes.c_double]
         L.svo_measure_all_outcome.restype = ctypes.c_uint64
+        L.svo_measure_all.argtypes = [ctypes.c_int, dp, ctypes.c_double]
+        L.svo_measure_all.restype = ctypes.c_uint64
+        L.svo_measure_qubit.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_double]
+        L.svo_measure_qubit.restype = ctypes.c_int
+        L.svo_measure_partial.argtypes = [ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                          ctypes.c_double]
+        L.svo_measure_partial.restype = ctypes.c_uint64
         L.svo_splitmix64.argtypes = [ctypes.c_uint64]
         L.svo_splitmix64.restype = ctypes.c_uint64
         L.svo_qgtl_shots.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_uint64, u64p]
@@ -156,6 +163,17 @@ class Oracle:
 
     def measure_all_outcome(self, n, amps, u):
         return int(self.L.svo_measure_all_outcome(n, _ptr(amps.view(np.float64)), u))
+
+    # collapsing measurements: amps (complex128, C-contiguous) is updated in place
+    def measure_all(self, n, amps, u):
+        return int(self.L.svo_measure_all(n, _ptr(amps.view(np.float64)), u))
+
+    def measure_qubit(self, n, amps, q, u):
+        return int(self.L.svo_measure_qubit(n, _ptr(amps.view(np.float64)), q, u))
+
+    def measure_partial(self, n, amps, qubits, u):
+        qs = (ctypes.c_int * len(qubits))(*qubits)
+        return int(self.L.svo_measure_partial(n, _ptr(amps.view(np.float64)), qs, len(qubits), u))
 
     def splitmix64(self, x):
         return int(self.L.svo_splitmix64(x))
@@ -246,6 +264,10 @@ class RefLib:
         L.measurement_correlation_zz.restype = ctypes.c_double
         L.measurement_all_qubits.argtypes = [vp, ctypes.c_double]
         L.measurement_all_qubits.restype = ctypes.c_uint64
+        L.measurement_single_qubit.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.measurement_single_qubit.restype = ctypes.c_int
+        L.measurement_partial.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_double]
+        L.measurement_partial.restype = ctypes.c_uint64
         L.quantum_state_clone.argtypes = [vp, vp]
         L.quantum_state_clone.restype = ctypes.c_int
         ip = ctypes.c_int

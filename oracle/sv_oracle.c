@@ -575,6 +575,101 @@ uint64_t svo_measure_all_outcome(int n, const double *a, double u)
     return have_nz ? last_nz : 0;
 }
 
+/* Collapsing measurements (measurement.c:181-231, 263-307, 320-410).  Norm
+ * floor MEASUREMENT_SMALL_NORM = 1e-15 (measurement.c:33). */
+#define SVO_SMALL_NORM 1e-15
+
+uint64_t svo_measure_all(int n, double *a, double u)
+{
+    /* measurement_all_qubits: the outcome rule above, then |outcome> with
+     * amplitude exactly 1 (measurement.c:300-302). */
+    const uint64_t r = svo_measure_all_outcome(n, a, u);
+    const uint64_t dim = 1ULL << n;
+    for (uint64_t i = 0; i < dim; i++) {
+        RE(a, i) = i == r ? 1.0 : 0.0;
+        IM(a, i) = 0.0;
+    }
+    return r;
+}
+
+int svo_measure_qubit(int n, double *a, int q, double u)
+{
+    /* measurement_single_qubit: outcome = u < P(1) (P(1) un-normalized,
+     * measurement.c:197-200); zero the other half while summing the kept
+     * half's cabs^2 in index order; below the norm floor return -1 with the
+     * state left as the reference leaves it (zeroed, not rescaled); else
+     * scale by 1 / sqrt(norm) (measurement.c:201-228). */
+    if (!qubit_ok(n, q)) return -1;
+    const uint64_t dim = 1ULL << n, mk = 1ULL << q;
+    const int r = u < svo_probability_one(n, a, q) ? 1 : 0;
+    double nf = 0.0;
+    for (uint64_t i = 0; i < dim; i++) {
+        if (((i & mk) ? 1 : 0) != r) {
+            RE(a, i) = 0.0;
+            IM(a, i) = 0.0;
+        } else {
+            nf += prob_cabs(a, i);
+        }
+    }
+    if (nf <= SVO_SMALL_NORM) return -1;
+    const double inv = 1.0 / sqrt(nf);
+    for (uint64_t i = 0; i < dim; i++) {
+        RE(a, i) *= inv;
+        IM(a, i) *= inv;
+    }
+    return r;
+}
+
+uint64_t svo_measure_partial(int n, double *a, const int *qubits, int k, double u)
+{
+    /* measurement_partial: outcome bit j = qubit qubits[j]; outcome
+     * probabilities accumulated in index order, the cumulative rule with the
+     * last-nonzero fallback, then zero / renormalize (renormalization skipped
+     * at or below the floor); 0 for invalid arguments (measurement.c:320-410). */
+    if (k <= 0 || k > 30) return 0;
+    for (int j = 0; j < k; j++)
+        if (!qubit_ok(n, qubits[j])) return 0;
+    const uint64_t dim = 1ULL << n, no = 1ULL << k;
+    double *pr = (double *)calloc(no, sizeof(double));
+    if (!pr) return 0;
+    for (uint64_t i = 0; i < dim; i++) {
+        uint64_t o = 0;
+        for (int j = 0; j < k; j++)
+            if (i & (1ULL << qubits[j])) o |= 1ULL << j;
+        pr[o] += prob_cabs(a, i);
+    }
+    double cum = 0.0;
+    uint64_t r = 0, last_nz = 0;
+    int found = 0, have_nz = 0;
+    for (uint64_t o = 0; o < no; o++) {
+        if (pr[o] > 0.0) { last_nz = o; have_nz = 1; }
+        cum += pr[o];
+        if (u < cum) { r = o; found = 1; break; }
+    }
+    if (!found) r = have_nz ? last_nz : 0;
+    double nf = 0.0;
+    for (uint64_t i = 0; i < dim; i++) {
+        uint64_t o = 0;
+        for (int j = 0; j < k; j++)
+            if (i & (1ULL << qubits[j])) o |= 1ULL << j;
+        if (o != r) {
+            RE(a, i) = 0.0;
+            IM(a, i) = 0.0;
+        } else {
+            nf += prob_cabs(a, i);
+        }
+    }
+    if (nf > SVO_SMALL_NORM) {
+        const double inv = 1.0 / sqrt(nf);
+        for (uint64_t i = 0; i < dim; i++) {
+            RE(a, i) *= inv;
+            IM(a, i) *= inv;
+        }
+    }
+    free(pr);
+    return r;
+}
+
 /* ---------------------------------------------------------------------------
  * Job-path PRNGs and shot protocol.
  * ------------------------------------------------------------------------- */

@@ -76,6 +76,14 @@ double svo_fidelity(int n, const double *a, const double *b);
  * (measurement.c:263-307 outcome rule, without the collapse write). */
 uint64_t svo_measure_all_outcome(int n, const double *amps, double u);
 
+/* Collapsing measurements: measurement_all_qubits (-> |outcome>),
+ * measurement_single_qubit (outcome or -1 below the norm floor) and
+ * measurement_partial (outcome bits in the order of `qubits`),
+ * measurement.c:181-231, 263-307, 320-410. */
+uint64_t svo_measure_all(int n, double *amps, double u);
+int svo_measure_qubit(int n, double *amps, int q, double u);
+uint64_t svo_measure_partial(int n, double *amps, const int *qubits, int k, double u);
+
 /* PRNGs of the job path. */
 uint64_t svo_splitmix64(uint64_t x);
 uint64_t svo_xorshift64(uint64_t *s);

@@ -627,3 +627,68 @@ def test_gate_families_at_33_and_34_qubits(eng, n):
         z, _ = s.expect_z_all()
     assert abs(head[0] - 1.0) < 1e-12 and abs(head[1]) < 1e-12
     assert np.max(np.abs(z - 1.0)) < 1e-12
+
+
+def _state_of(s):
+    return s.read(0, 1 << s.n).astype(np.complex128)
+
+
+@pytest.mark.parametrize("n", [5, 14, 20])
+def test_collapsing_measurements_vs_oracle(eng, oracle, n):
+    """measure_qubit / measure_partial / measure_all against the oracle's
+    restatement of measurement_single_qubit / _partial / _all_qubits
+    (complex128): the same outcome, the same zeroed amplitudes, the kept ones
+    within 1e-12; normalized and unnormalized states, draws at and past the
+    total probability, several qubit orders."""
+    rng = np.random.default_rng(40 + n)
+    base = random_state(n, rng)
+    qsets = [[0], [n - 1], [2, 0], [n - 1, 1, 3], [4, n - 2, 0, 2, 1]]
+    for a in (base, base * 1.7):
+        for u in (0.0, 0.3, 0.77, 1.0 - 2 ** -53, 1.0, 1.5):
+            for q in (0, n // 2, n - 1):
+                want_a = a.copy()
+                want = oracle.measure_qubit(n, want_a, q, u)
+                with eng.StateVector(n) as s:
+                    got = s.load(a).measure_qubit(q, u)
+                    got_a = _state_of(s)
+                assert got == want, (q, u)
+                assert np.array_equal(got_a == 0, want_a == 0) and np.max(np.abs(got_a - want_a)) < 1e-12
+            for qs in qsets:
+                want_a = a.copy()
+                want = oracle.measure_partial(n, want_a, qs, u)
+                with eng.StateVector(n) as s:
+                    got = s.load(a).measure_partial(qs, u)
+                    got_a = _state_of(s)
+                assert got == want, (qs, u)
+                assert np.array_equal(got_a == 0, want_a == 0) and np.max(np.abs(got_a - want_a)) < 1e-12
+            want_a = a.copy()
+            want = oracle.measure_all(n, want_a, u)
+            with eng.StateVector(n) as s:
+                got = s.load(a).measure_all(u)
+                got_a = _state_of(s)
+            assert got == want and np.array_equal(got_a, want_a), u
+
+
+def test_collapsing_measurement_edge_cases(eng):
+    """A zero-weight single-qubit outcome gives -1 and leaves the state zeroed
+    (measurement.c:219-221); repeated and out-of-range qubits are refused;
+    complex64 states collapse within 1e-6 and renormalize to 1."""
+    rng = np.random.default_rng(5)
+    n = 9
+    a = random_state(n, rng)
+    a[np.arange(1 << n) & 4 == 0] = 0.0     # qubit 2 is surely 1
+    a /= np.linalg.norm(a)
+    with eng.StateVector(n) as s:
+        assert s.load(a).measure_qubit(2, 1.5) == -1
+        assert not np.any(_state_of(s))
+        with pytest.raises(eng.EngineError):
+            s.measure_partial([1, 1], 0.5)
+        with pytest.raises(eng.EngineError):
+            s.measure_qubit(n, 0.5)
+    b = random_state(n, rng)
+    with eng.StateVector(n, eng.C64) as s:
+        r = s.load(b).measure_partial([3, 0], 0.6)
+        got = _state_of(s)
+        _, nrm = s.expect_z_all()
+    keep = ((np.arange(1 << n) >> 3) & 1) | (((np.arange(1 << n) >> 0) & 1) << 1)
+    assert np.all(got[keep != r] == 0) and abs(nrm - 1.0) < 1e-5

@@ -12,6 +12,8 @@ fails it:
 import json
 import os
 
+import ctypes
+
 import numpy as np
 import pytest
 import scipy.linalg
@@ -108,6 +110,67 @@ def test_oracle_collapse_rule_matches_reference(oracle, ref):
         expect = ref.L.measurement_all_qubits(s, u)
         ref.free(s)
         assert oracle.measure_all_outcome(n, a, u) == expect, u
+
+
+def _ref_state_from(ref, n, a):
+    s = ref.L.ref_state_new(n)
+    amp = ref.L.ref_state_amplitudes(s)
+    for i in range(1 << n):
+        amp[2 * i] = a[i].real
+        amp[2 * i + 1] = a[i].imag
+    return s
+
+
+def _ref_amplitudes(ref, s, n):
+    amp = ref.L.ref_state_amplitudes(s)
+    return np.array([complex(amp[2 * i], amp[2 * i + 1]) for i in range(1 << n)])
+
+
+def test_oracle_collapsing_measurements_match_reference(oracle, ref):
+    """measurement_single_qubit / measurement_partial / measurement_all_qubits
+    of the compiled reference against the oracle on the same states: the
+    outcome exactly, the zeroed amplitudes exactly, the renormalized ones
+    within 1e-15 (the reference build contracts its norm sum into FMAs), for
+    draws at and past the total probability, an unnormalized state, a
+    zero-weight outcome (single qubit: -1 below the 1e-15 floor) and several
+    qubit orders."""
+    rng = np.random.default_rng(31)
+    n = 7
+    base = oracle.run(n, random_gate_list(n, 40, rng))
+    states = [base, base * 1.7]                      # normalized and not
+    z = base.copy()
+    z[np.arange(1 << n) & 8 == 0] = 0.0              # qubit 3 is surely 1
+    z /= np.linalg.norm(z)
+    states.append(z)
+    draws = [0.0, 0.2, 0.5, 0.77, 0.999999, 1.0 - 2 ** -53, 1.0, 1.5]
+    for a in states:
+        for u in draws:
+            for q in (0, 3, 6):
+                s = _ref_state_from(ref, n, a)
+                want = ref.L.measurement_single_qubit(s, q, u)
+                ra = _ref_amplitudes(ref, s, n)
+                ref.free(s)
+                b = a.copy()
+                assert oracle.measure_qubit(n, b, q, u) == want, (q, u)
+                assert np.array_equal(b == 0, ra == 0) and np.max(np.abs(b - ra)) < 1e-15, (q, u)
+            for qs in ([2], [0, 5], [6, 1, 3], [4, 2, 0, 6]):
+                s = _ref_state_from(ref, n, a)
+                want = ref.L.measurement_partial(s, (ctypes.c_int * len(qs))(*qs), len(qs), u)
+                ra = _ref_amplitudes(ref, s, n)
+                ref.free(s)
+                b = a.copy()
+                assert oracle.measure_partial(n, b, qs, u) == want, (qs, u)
+                assert np.array_equal(b == 0, ra == 0) and np.max(np.abs(b - ra)) < 1e-15, (qs, u)
+            s = _ref_state_from(ref, n, a)
+            want = ref.L.measurement_all_qubits(s, u)
+            ra = _ref_amplitudes(ref, s, n)
+            ref.free(s)
+            b = a.copy()
+            assert oracle.measure_all(n, b, u) == want and np.array_equal(b, ra), u
+    # a draw past P(1) = 1 picks the zero-weight outcome 0: -1 and, as the
+    # reference leaves it, the state zeroed
+    b = z.copy()
+    assert oracle.measure_qubit(n, b, 3, 1.5) == -1 and not np.any(b)
 
 
 # ---------------------------------------------------------------------------
