@@ -284,6 +284,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_device_count": ([], ip),
         "moonlab_hip_last_error": ([], ctypes.c_char_p),
         "moonlab_hip_set_gate_tuning": ([ip, ip], ip),
+        "moonlab_hip_set_blocked_lookahead": ([ip], ip),
         "moonlab_hip_state_create": ([u32, ctypes.POINTER(vp)], ip),
         "moonlab_hip_state_create_ex": ([u32, ip, ctypes.POINTER(vp)], ip),
         "moonlab_hip_state_free": ([vp], None),
@@ -879,6 +880,11 @@ def set_blocked_engine(engine: str) -> None:
 
 def set_streaming_threshold(nbytes: int) -> None:
     _check_hip(lib().moonlab_hip_set_streaming_threshold(nbytes), "set_streaming_threshold")
+
+
+def set_blocked_lookahead(windows: int = 32) -> None:
+    """Candidate tile-set windows per pass of the blocked planner (0: greedy)."""
+    _check_hip(lib().moonlab_hip_set_blocked_lookahead(windows), "set_blocked_lookahead")
 
 
 def set_gate_tuning(pairs_per_thread: int = 0, block_threads: int = 0) -> None:

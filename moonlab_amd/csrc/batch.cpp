@@ -398,7 +398,7 @@ uint64_t touches(const GOp &o)
 
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
-                 std::vector<BOp> &ops, int anchor_bits, int amp_bytes, int reg_bits)
+                 std::vector<BOp> &ops, int anchor_bits, int amp_bytes, int reg_bits, int lookahead)
 {
     const bool resident = n <= tb;
     // low qubits kept in every tile (runs of 2^anchor_bits contiguous
@@ -459,24 +459,57 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
     // op whose non-diagonal targets fit the tile set and that shares no qubit
     // with an op already deferred from this pass; the deferred ops, still in
     // circuit order, make up the rest of the circuit for the next pass.
-    std::vector<const GOp *> rest, later;
+    //
+    // With lookahead > 0 the tile set is chosen before the scan: besides the
+    // set the scan grows from the anchor alone, the first `lookahead`
+    // distinct windows seeded by pending ops (the anchor, the seed's target
+    // and the qubits above it, then below) are scanned, and the set that
+    // takes the most ops wins (the first such, in seed order).  A window of
+    // neighbouring qubits lets one pass run many layers of a nearest-
+    // neighbour circuit inside it.
+    std::vector<const GOp *> rest, later, best_cur, best_later, try_cur, try_later;
     rest.reserve(gops.size());
     for (const GOp &o : gops) rest.push_back(&o);
-    while (!rest.empty()) {
-        uint64_t blocked = 0;
-        later.clear();
+    auto scan = [&](uint64_t S0, std::vector<const GOp *> &take, std::vector<const GOp *> &defer) {
+        uint64_t Sx = S0, blocked = 0;
+        take.clear();
+        defer.clear();
         for (const GOp *o : rest) {
             const uint64_t t = touches(*o), need = needs_local(*o);
-            if (!(t & blocked) && __builtin_popcountll(S | need) <= tb) {
-                S |= need;
-                cur.push_back(o);
+            if (!(t & blocked) && __builtin_popcountll(Sx | need) <= tb) {
+                Sx |= need;
+                take.push_back(o);
             } else {
                 blocked |= t;
-                later.push_back(o);
+                defer.push_back(o);
             }
         }
+        return Sx;
+    };
+    std::vector<uint64_t> tried;
+    while (!rest.empty()) {
+        uint64_t bestS = scan(anchor_mask, best_cur, best_later);
+        tried.clear();
+        for (size_t i = 0; i < rest.size() && (int)tried.size() < lookahead; i++) {
+            const uint64_t nd = needs_local(*rest[i]) & ~anchor_mask;
+            if (!nd) continue;
+            uint64_t S0 = anchor_mask | nd;
+            const int q = __builtin_ctzll(nd);
+            for (int k = q; k < n && __builtin_popcountll(S0) < tb; k++) S0 |= 1ull << k;
+            for (int k = q - 1; k >= 0 && __builtin_popcountll(S0) < tb; k--) S0 |= 1ull << k;
+            if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
+            tried.push_back(S0);
+            const uint64_t Sx = scan(S0, try_cur, try_later);
+            if (try_cur.size() > best_cur.size()) {
+                bestS = Sx;
+                best_cur.swap(try_cur);
+                best_later.swap(try_later);
+            }
+        }
+        S = bestS;
+        cur = best_cur;
         close();
-        rest.swap(later);
+        rest.swap(best_later);
     }
 }
 
@@ -854,7 +887,7 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     int anchor = 4, reg_bits = kRegBits;
     const int tb = blocked_shape(n, prec, tune, &reg, &anchor, &reg_bits);
     plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor,
-                prec == MOONLAB_HIP_C64 ? 8 : 16, reg_bits);
+                prec == MOONLAB_HIP_C64 ? 8 : 16, reg_bits, tune.blocked_lookahead);
     if (ops.empty()) return MOONLAB_HIP_OK;
     hipStream_t st = (hipStream_t)moonlab_hip_state_stream(s);
     static std::atomic<int> cached_cus{0};   // shared by every launching thread
@@ -892,7 +925,7 @@ int moonlab_hip_blocked_pass_count(const moonlab_hip_gate_t *g, int ng, int n, i
     int anchor = 4, reg_bits = kRegBits;
     const int tb = blocked_shape(n, precision, gate_tuning(), &reg, &anchor, &reg_bits);
     plan_passes(gops, n, tb, reg ? kRegPhases : kSegRegsPass, passes, ops, anchor,
-                precision == MOONLAB_HIP_C64 ? 8 : 16, reg_bits);
+                precision == MOONLAB_HIP_C64 ? 8 : 16, reg_bits, gate_tuning().blocked_lookahead);
     return (int)passes.size();
 }
 

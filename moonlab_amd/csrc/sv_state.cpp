@@ -190,6 +190,13 @@ moonlab_hip_status_t moonlab_hip_set_blocked_tuning(int tile_bits, int threads)
     return MOONLAB_HIP_OK;
 }
 
+moonlab_hip_status_t moonlab_hip_set_blocked_lookahead(int windows)
+{
+    if (windows < 0 || windows > 1024) return fail(MOONLAB_HIP_ERR_BAD_ARG, "lookahead windows 0..1024");
+    gate_tuning().blocked_lookahead = windows;
+    return MOONLAB_HIP_OK;
+}
+
 moonlab_hip_status_t moonlab_hip_set_blocked_engine(int engine)
 {
     if (engine < 0 || engine > 3) return fail(MOONLAB_HIP_ERR_BAD_ARG, "engine is 0, 1, 2 or 3");

@@ -184,7 +184,7 @@ int main() {
             int tbb = tb < n ? tb : n;
             std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
             std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
-            mlx::plan_passes(gops, n, tbb, sr, passes, ops);
+            mlx::plan_passes(gops, n, tbb, sr, passes, ops, 4, 16, mlx::kRegBits, (trial & 1) ? mlx::kBlockedLookahead : 0);
             std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
             for (const mlx::BPass &P : passes) {
                 int T = 1 << P.nbits; uint64_t ntiles = 1ull << (n - P.nbits);
@@ -236,7 +236,9 @@ int main() {
             std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
             std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
             const int rb = 3 + ((trial >> 1) & 1);   // both register widths with both amplitude widths
-            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1), (trial & 1) ? 8 : 16, rb);
+            // lookahead windows on two trials in three
+            mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3 + (trial & 1), (trial & 1) ? 8 : 16, rb,
+                             trial % 3 ? mlx::kBlockedLookahead : 0);
             std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
             for (const mlx::BPass &P : passes) {
                 if (P.nbits != tb) { printf("register pass with %d tile bits\n", P.nbits); return 1; }
@@ -255,6 +257,41 @@ int main() {
             if (err > 1e-12) { printf("REGISTER PHASE MISMATCH n=%d tb=%d err=%g\n", n, tb, err); return 1; }
             reg_checked++;
         }
+    }
+    // nearest-neighbour layers on 18 qubits (rotations + CNOT brickwork), the
+    // circuits whose passes the lookahead windows change the most
+    for (int trial = 0; trial < 2; trial++) {
+        const int n = 18, tb = 12;
+        std::vector<moonlab_hip_gate_t> g;
+        for (int q = 0; q < n; q++) g.push_back({4, q, -1, -1, 0, 0, 0});
+        for (int d = 0; d < 16; d++) {
+            for (int q = 0; q < n; q++)
+                g.push_back({7 + (int)(rng() % 3), q, -1, -1, (double)(rng() % 1000) / 77.0, 0, 0});
+            for (int q = 0; q + 1 < n; q += 2) g.push_back({10, q + 1, q, -1, 0, 0, 0});
+            for (int q = 1; q + 1 < n; q += 2) g.push_back({10, q + 1, q, -1, 0, 0, 0});
+        }
+        std::vector<double> want(2ull << n, 0.0); want[0] = 1.0;
+        svo_apply_gates(n, want.data(), (const svo_gate_t *)g.data(), (int)g.size());
+        std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
+        std::vector<mlx::BPass> passes, greedy; std::vector<mlx::BOp> ops, gops2;
+        mlx::plan_passes(gops, n, tb, mlx::kRegPhases, greedy, gops2, 3, 16, 4, 0);
+        mlx::plan_passes(gops, n, tb, mlx::kRegPhases, passes, ops, 3, 16, 4, mlx::kBlockedLookahead);
+        if (passes.size() >= greedy.size()) { printf("lookahead planned %zu passes, greedy %zu\n", passes.size(), greedy.size()); return 1; }
+        std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
+        for (const mlx::BPass &P : passes) {
+            int T = 1 << P.nbits; uint64_t ntiles = 1ull << (n - P.nbits);
+            for (uint64_t ti = 0; ti < ntiles; ti++) {
+                uint64_t base = ti; for (int b = 0; b < P.nbits; b++) base = ins0(base, P.qubits[b]);
+                std::vector<cd> t(T);
+                auto addr = [&](int l) { uint64_t a = base; for (int b = 0; b < P.nbits; b++) if (l & (1 << b)) a |= 1ull << P.qubits[b]; return a; };
+                for (int l = 0; l < T; l++) t[l] = st[addr(l)];
+                reg_tile<16>(t, P.nbits, 4, &ops[P.first_op], P.num_ops, base, relayouts);
+                for (int l = 0; l < T; l++) st[addr(l)] = t[l];
+            }
+        }
+        double err = 0; for (uint64_t i = 0; i < (1ull << n); i++) err = std::max(err, std::abs(st[i] - cd(want[2 * i], want[2 * i + 1])));
+        if (err > 1e-12) { printf("LOOKAHEAD PLAN MISMATCH err=%g\n", err); return 1; }
+        reg_checked++;
     }
     printf("ok: %d plans checked, %d segment executions, %d register-phase plans, %d layout changes\n", checked,
            segs_seen, reg_checked, relayouts);
