@@ -470,7 +470,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
-    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--depth", type=int, default=None,
+                    help="layers of the random-layer circuit (default 40 for the sweep, 10 for --workload blocked)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -478,6 +479,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-count", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-sweep", default="50,200,400",
+                    help="other arrival rates (per s) at which the served leg runs again")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--workload", choices=("sweep", "fusion", "blocked"), default="sweep",
                     help="sweep: the 30-qubit gate sweep (with the batch and serving legs); "
@@ -498,7 +501,7 @@ def main() -> int:
 
     if args.workload == "blocked":
         t0 = time.perf_counter()
-        depth = args.depth if args.depth != 40 else 10
+        depth = args.depth if args.depth is not None else 10
         res = run_blocked_workload(m, args.qubits, depth)
         wall = time.perf_counter() - t0
         c128 = res["c128"]
@@ -540,6 +543,8 @@ def main() -> int:
         return 0
 
     n = args.qubits
+    if args.depth is None:
+        args.depth = 40
     gates = workloads.random_layer_circuit(n, args.depth, SEED, h_layer=True)
     G = len(gates)
     bytes_step = workloads.algorithmic_bytes(gates, n)
@@ -621,11 +626,23 @@ def main() -> int:
         if not args.no_cpu:
             cpu_serve = serving_cpu_baseline(workloads.serving_requests(args.serve_count, SEED + 5, args.serve_rate),
                                              15.0)
+        # the same request stream at other arrival rates, served only: plain
+        # dispatch runs one request at a time, so its circuits/s does not
+        # depend on the arrival rate once arrivals outpace it (~9/s)
+        by_rate = {}
+        for r in [float(x) for x in args.serve_sweep.split(",") if x.strip()]:
+            if r == args.serve_rate:
+                continue
+            sv = run_serving_workload(m, args.serve_count, r, False)
+            by_rate[f"{r:g}"] = {"circuits_per_s": sv["circuits_per_s"], "p50_ms": sv["p50_ms"],
+                                 "p99_ms": sv["p99_ms"], "makespan_s": sv["makespan_s"], "failed": sv["failed"],
+                                 "ratio_vs_plain": round(sv["circuits_per_s"] / plain["circuits_per_s"], 2)}
         serving = {"workload": f"{args.serve_count} Poisson requests at {args.serve_rate:g}/s: random-layer "
                                "circuits, n in 10..30 with P(n) ~ 2^(-(n-10)/4), depth ~ U{10..50}, "
                                "complex128/complex64 50/50, 1024 shots + <Z_q> each, 256 GiB HBM budget",
                    "served": served, "plain_sequential": plain,
                    "circuits_per_s_ratio": round(served["circuits_per_s"] / plain["circuits_per_s"], 2),
+                   "served_at_other_rates": by_rate,
                    "cpu_baseline": cpu_serve}
     out = {
         "metric": "HBM TB/s on 30q gate sweep",
