@@ -237,7 +237,8 @@ void segment_ops(const std::vector<BOp> &in, int nbits, int seg_regs, std::vecto
 // first phase tries the IO layout (the top reg_bits tile bits), which needs
 // no LDS exchange; a register set is filled up with the highest free bits.
 // SWAP lowers to three exchanges, X_b[a] X_a[b] X_b[a], its controls kept.
-void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int reg_bits, std::vector<BOp> &out)
+void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int reg_bits, std::vector<BOp> &out,
+                     int lookahead)
 {
     const int nv = 1 << reg_bits;
     std::vector<BOp> in;
@@ -302,10 +303,39 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int 
                 S = 0;
                 scan(false);
             }
-            first = false;
         } else {
             scan(false);
         }
+        // lookahead: register sets seeded by pending ops (the op's bits, then
+        // the bits above its lowest one, then below) compete with the set
+        // above; the one taking the most ops wins (the IO layout of the first
+        // phase, which needs no layout change, on ties)
+        if (lookahead > 0) {
+            uint32_t bestS = S;
+            std::vector<const BOp *> best_cur = cur, best_later = later;
+            std::vector<uint32_t> tried;
+            for (size_t i = 0; i < rest.size() && (int)tried.size() < lookahead; i++) {
+                const uint32_t nd = need(*rest[i]);
+                if (!nd || __builtin_popcount(nd) > reg_bits) continue;
+                uint32_t S0 = nd;
+                const int q = __builtin_ctz(nd);
+                for (int k = q; k < nbits && __builtin_popcount(S0) < reg_bits; k++) S0 |= 1u << k;
+                for (int k = q - 1; k >= 0 && __builtin_popcount(S0) < reg_bits; k--) S0 |= 1u << k;
+                if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
+                tried.push_back(S0);
+                S = S0;
+                scan(true);
+                if (cur.size() > best_cur.size()) {
+                    bestS = S;
+                    best_cur.swap(cur);
+                    best_later.swap(later);
+                }
+            }
+            S = bestS;
+            cur.swap(best_cur);
+            later.swap(best_later);
+        }
+        first = false;
         for (int q = nbits - 1; q >= 0 && __builtin_popcount(S) < reg_bits; q--) S |= 1u << q;
         int r[kRegBits];
         for (int q = 0, k = 0; q < nbits; q++)
@@ -443,7 +473,7 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
             memcpy(b.m, o->m, sizeof b.m);
             pass_ops.push_back(b);
         }
-        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, amp_bytes, reg_bits, ops);
+        if (seg_regs == kRegPhases) plan_reg_phases(pass_ops, P.nbits, amp_bytes, reg_bits, ops, lookahead);
         else segment_ops(pass_ops, P.nbits, seg_regs, ops);
         P.num_ops = (int)ops.size() - P.first_op;
         passes.push_back(P);
