@@ -882,8 +882,8 @@ def set_streaming_threshold(nbytes: int) -> None:
     _check_hip(lib().moonlab_hip_set_streaming_threshold(nbytes), "set_streaming_threshold")
 
 
-def set_blocked_lookahead(windows: int = 32) -> None:
-    """Candidate tile-set windows per pass of the blocked planner (0: greedy)."""
+def set_blocked_lookahead(windows: int = 8) -> None:
+    """Seed ops per pass and phase of the blocked planner's lookahead (0: greedy)."""
     _check_hip(lib().moonlab_hip_set_blocked_lookahead(windows), "set_blocked_lookahead")
 
 

@@ -491,11 +491,12 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
     // circuit order, make up the rest of the circuit for the next pass.
     //
     // With lookahead > 0 the tile set is chosen before the scan: besides the
-    // set the scan grows from the anchor alone, the first `lookahead`
-    // distinct windows seeded by pending ops (the anchor, the seed's target
-    // and the qubits above it, then below) are scanned, and the set that
-    // takes the most ops wins (the first such, in seed order).  A window of
-    // neighbouring qubits lets one pass run many layers of a nearest-
+    // set the scan grows from the anchor alone, windows seeded by the first
+    // `lookahead` pending ops with a target outside the anchor are scanned
+    // (the anchor, the seed's target, and the free qubits in a contiguous
+    // run starting at, centred on or ending at that target), and the set
+    // that takes the most ops wins (the first such, in seed order).  A window
+    // of neighbouring qubits lets one pass run many layers of a nearest-
     // neighbour circuit inside it.
     std::vector<const GOp *> rest, later, best_cur, best_later, try_cur, try_later;
     rest.reserve(gops.size());
@@ -517,23 +518,30 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
         return Sx;
     };
     std::vector<uint64_t> tried;
+    const int free_bits = tb - __builtin_popcountll(anchor_mask);
     while (!rest.empty()) {
         uint64_t bestS = scan(anchor_mask, best_cur, best_later);
         tried.clear();
-        for (size_t i = 0; i < rest.size() && (int)tried.size() < lookahead; i++) {
+        int seeds = 0;
+        for (size_t i = 0; i < rest.size() && seeds < lookahead; i++) {
             const uint64_t nd = needs_local(*rest[i]) & ~anchor_mask;
             if (!nd) continue;
-            uint64_t S0 = anchor_mask | nd;
+            seeds++;
             const int q = __builtin_ctzll(nd);
-            for (int k = q; k < n && __builtin_popcountll(S0) < tb; k++) S0 |= 1ull << k;
-            for (int k = q - 1; k >= 0 && __builtin_popcountll(S0) < tb; k--) S0 |= 1ull << k;
-            if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
-            tried.push_back(S0);
-            const uint64_t Sx = scan(S0, try_cur, try_later);
-            if (try_cur.size() > best_cur.size()) {
-                bestS = Sx;
-                best_cur.swap(try_cur);
-                best_later.swap(try_later);
+            // windows starting at, centred on and ending at the seed's target
+            for (const int lo0 : {q, q - free_bits / 2, q - free_bits + 1}) {
+                uint64_t S0 = anchor_mask | nd;
+                const int lo = lo0 < 0 ? 0 : lo0;
+                for (int k = lo; k < n && __builtin_popcountll(S0) < tb; k++) S0 |= 1ull << k;
+                for (int k = lo - 1; k >= 0 && __builtin_popcountll(S0) < tb; k--) S0 |= 1ull << k;
+                if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
+                tried.push_back(S0);
+                const uint64_t Sx = scan(S0, try_cur, try_later);
+                if (try_cur.size() > best_cur.size()) {
+                    bestS = Sx;
+                    best_cur.swap(try_cur);
+                    best_later.swap(try_later);
+                }
             }
         }
         S = bestS;

@@ -30,11 +30,12 @@ bool lower(const moonlab_hip_gate_t *g, int ng, int n, std::vector<GOp> &out);
 // kRegPhases, into a register-phase program (BOP_PHASE) of `reg_bits`
 // register bits per phase for amplitudes of `amp_bytes` bytes (16
 // complex128, 8 complex64; the phase headers carry LDS slot offsets).
-// lookahead > 0 picks each pass's tile set among that many candidate
-// windows of neighbouring qubits (the set taking the most ops); 0 grows it
-// greedily in circuit order.
+// lookahead > 0 picks each pass's tile set among windows of neighbouring
+// qubits seeded by that many pending ops (three windows each), and each
+// phase's register set among sets seeded by as many pending ops (the set
+// taking the most ops); 0 grows both greedily in circuit order.
 constexpr int kRegPhases = -1;
-constexpr int kBlockedLookahead = 32;
+constexpr int kBlockedLookahead = 8;
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
                  std::vector<BOp> &ops, int anchor_bits = 4, int amp_bytes = 16, int reg_bits = kRegBits,
                  int lookahead = 0);

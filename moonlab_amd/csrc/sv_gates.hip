@@ -27,8 +27,8 @@ GateTuning &gate_tuning()
     // every gate family at 30 qubits (2, 4, 8 pairs per thread: 2-10 % slower).
     // Blocked path: tile bits 0 = by precision (12 for complex128, 13 for
     // complex64), 512-thread LDS-engine workgroups, register-phase engine,
-    // 32 candidate tile-set windows per pass.
-    static GateTuning t = {1, 256, 512ull << 20, 0, 512, 0, 32};
+    // lookahead from 8 seed ops per pass and phase.
+    static GateTuning t = {1, 256, 512ull << 20, 0, 512, 0, 8};
     return t;
 }
 

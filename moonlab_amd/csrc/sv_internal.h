@@ -29,7 +29,7 @@ struct GateTuning {
     int blocked_threads = 512;    // workgroup size of its LDS op engine: 256 or 512
     int blocked_engine = 0;       // 0, 2: register phases (k_reg_pass), 16 values per thread; 1: LDS
                                   // ops (k_tile_pass); 3: register phases of 8 values per thread
-    int blocked_lookahead = 32;   // candidate tile-set windows per pass of the blocked planner (0: greedy)
+    int blocked_lookahead = 8;    // seed ops per pass / phase of the blocked planner's lookahead (0: greedy)
 };
 GateTuning &gate_tuning();
 
