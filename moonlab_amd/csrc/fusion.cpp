@@ -262,6 +262,103 @@ std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kma
     return blocks;
 }
 
+// Window planning: gates commute when they share no qubit, so a block takes,
+// in circuit order, every pending gate whose qubits fit its qubit set and
+// that shares no qubit with a gate already passed over; the passed-over
+// gates, still in order, are the rest of the circuit.  The set is chosen by
+// lookahead: besides the set grown from the first pending gate, the first
+// `seeds` pending gates seed windows of kmax neighbouring qubits (starting
+// at, centred on and ending at the gate's lowest qubit), and the set taking
+// the most gates wins.  On nearest-neighbour circuits a window runs several
+// layers inside its light cone, where the open-block planner closes a block
+// per layer.
+std::vector<FusedBlock> plan_fusion_windows(const moonlab_hip_gate_t *g, int ng, int kmax, int seeds, int n)
+{
+    std::vector<FusedBlock> blocks;
+    std::vector<const moonlab_hip_gate_t *> rest, best_take, best_defer, take, defer;
+    std::vector<uint64_t> qm((size_t)ng);
+    for (int i = 0; i < ng; i++) {
+        int q[3];
+        const int nq = gate_qubits(g[i], q);
+        uint64_t m = 0;
+        for (int j = 0; j < nq; j++) m |= 1ull << q[j];
+        qm[(size_t)i] = m;
+        rest.push_back(&g[i]);
+    }
+    auto mask = [&](const moonlab_hip_gate_t *x) { return qm[(size_t)(x - g)]; };
+    auto scan = [&](uint64_t S0, bool grow) {
+        uint64_t S = S0, blocked = 0;
+        take.clear();
+        defer.clear();
+        for (const moonlab_hip_gate_t *x : rest) {
+            const uint64_t m = mask(x);
+            const bool fits = grow ? __builtin_popcountll(S | m) <= kmax : (m & ~S) == 0;
+            if (!(m & blocked) && fits) {
+                S |= m;
+                take.push_back(x);
+            } else {
+                blocked |= m;
+                defer.push_back(x);
+            }
+        }
+    };
+    std::vector<uint64_t> tried;
+    while (!rest.empty()) {
+        scan(0, true);
+        best_take.swap(take);
+        best_defer.swap(defer);
+        tried.clear();
+        int used = 0;
+        for (size_t i = 0; i < rest.size() && used < seeds; i++) {
+            const uint64_t m = mask(rest[i]);
+            if (!m || __builtin_popcountll(m) > kmax) continue;
+            used++;
+            const int q = __builtin_ctzll(m);
+            for (const int lo0 : {q, q - kmax / 2, q - kmax + 1}) {
+                uint64_t S0 = m;
+                const int lo = lo0 < 0 ? 0 : lo0;
+                for (int k = lo; k < n && __builtin_popcountll(S0) < kmax; k++) S0 |= 1ull << k;
+                for (int k = lo - 1; k >= 0 && __builtin_popcountll(S0) < kmax; k--) S0 |= 1ull << k;
+                if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
+                tried.push_back(S0);
+                scan(S0, false);
+                if (take.size() > best_take.size()) {
+                    best_take.swap(take);
+                    best_defer.swap(defer);
+                }
+            }
+        }
+        if (best_take.empty()) {   // a gate wider than kmax stands alone
+            best_take.assign(rest.begin(), rest.begin() + 1);
+            best_defer.assign(rest.begin() + 1, rest.end());
+        }
+        FusedBlock b;
+        uint64_t u = 0;
+        for (const moonlab_hip_gate_t *x : best_take) u |= mask(x);
+        for (int q = 0; q < 64; q++)
+            if (u & (1ull << q)) b.qubits.push_back(q);
+        compose_block(b, best_take);
+        blocks.push_back(std::move(b));
+        rest.swap(best_defer);
+    }
+    return blocks;
+}
+
+// The open-block plan or the window plan, whichever has fewer blocks (each
+// block is one pass over the state).
+std::vector<FusedBlock> plan_fusion_best(const moonlab_hip_gate_t *g, int ng, int kmax)
+{
+    std::vector<FusedBlock> a = plan_fusion(g, ng, kmax);
+    int n = 0;
+    for (int i = 0; i < ng; i++) {
+        int q[3];
+        const int nq = gate_qubits(g[i], q);
+        for (int j = 0; j < nq; j++) n = std::max(n, q[j] + 1);
+    }
+    std::vector<FusedBlock> b = plan_fusion_windows(g, ng, kmax, kFusionSeeds, n);
+    return b.size() < a.size() ? b : a;
+}
+
 int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int kmax)
 {
     for (int i = 0; i < ng; i++) {
@@ -275,7 +372,7 @@ int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int
     }
     if (kmax < 1) kmax = 1;
     if (kmax > 5) kmax = 5;
-    const std::vector<FusedBlock> blocks = plan_fusion(g, ng, kmax);
+    const std::vector<FusedBlock> blocks = plan_fusion_best(g, ng, kmax);
     const SvDev d{moonlab_hip_state_device_ptr(s), (int)moonlab_hip_state_n_qubits(s), moonlab_hip_state_precision(s),
                   (hipStream_t)moonlab_hip_state_stream(s)};
     const bool f = d.prec == MOONLAB_HIP_C64;
@@ -368,7 +465,7 @@ int moonlab_hip_fusion_plan(const moonlab_hip_gate_t *g, int ng, int kmax, uint6
         for (int j = 0; j < nq; j++)
             if (q[j] < 0 || q[j] > 63 || (j > 0 && q[j] == q[j - 1])) return MOONLAB_HIP_ERR_INVALID_QUBIT;
     }
-    const std::vector<mlx::FusedBlock> blocks = mlx::plan_fusion(g, ng, kmax);
+    const std::vector<mlx::FusedBlock> blocks = mlx::plan_fusion_best(g, ng, kmax);
     for (int i = 0; i < (int)blocks.size() && i < max_blocks; i++) {
         const mlx::FusedBlock &b = blocks[(size_t)i];
         uint64_t m = 0;

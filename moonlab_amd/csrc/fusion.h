@@ -36,6 +36,13 @@ bool gate_matrix_1q(const moonlab_hip_gate_t &g, cd u[4]);
 // FP64 (left-multiplication in gate order).
 std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kmax);
 
+// Window planning (each block's qubit set chosen among windows of kmax
+// neighbouring qubits seeded by `seeds` pending gates), and the plan with
+// fewer blocks of the two; n bounds the windows.
+constexpr int kFusionSeeds = 8;
+std::vector<FusedBlock> plan_fusion_windows(const moonlab_hip_gate_t *g, int ng, int kmax, int seeds, int n);
+std::vector<FusedBlock> plan_fusion_best(const moonlab_hip_gate_t *g, int ng, int kmax);
+
 // Apply a gate list through fusion (returns a moonlab_hip_status_t value).
 int apply_fused(moonlab_hip_state_t *s, const moonlab_hip_gate_t *g, int ng, int kmax);
 

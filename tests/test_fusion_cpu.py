@@ -63,13 +63,39 @@ def test_plan_blocks_commute_across_disjoint_qubits():
 
 
 def test_plan_pass_count_on_su2_cz_depth60():
-    """32 qubits, depth 60: the planner keeps one pass per 4 qubits per layer
-    (an in-order planner needs 1320 passes here)."""
+    """32 qubits, depth 60: the open-block planner keeps one pass per 4
+    qubits per layer (480; an in-order planner needs 1320); windows of 5
+    neighbouring qubits run several layers in their light cone (316)."""
     import moonlab_amd as m
     from moonlab_amd import workloads
     gl = workloads.su2_cz_circuit(32, 60, 1)
     blocks = m.fusion_plan(gl, 5)
-    assert len(blocks) <= 480
+    assert len(blocks) <= 330
+    assert sum(b.num_gates for b in blocks) == len(gl)
+
+
+@pytest.mark.parametrize("kmax", [3, 4, 5])
+def test_window_plans_reproduce_brickwork_circuits(kmax, oracle):
+    """Nearest-neighbour SU(2) + CZ brickwork on 10 and 11 qubits, where the
+    window planner wins: the blocks, applied in order, give the circuit's
+    state (1e-12), and each block is unitary."""
+    import moonlab_amd as m
+    from moonlab_amd import workloads
+    rng = np.random.default_rng(7 + kmax)
+    for n in (10, 11):
+        gl = workloads.su2_cz_circuit(n, 12, 3 + n)
+        a0 = random_state(n, rng)
+        want = oracle.apply(n, a0.copy(), gl)
+        blocks = m.fusion_plan(gl, kmax)
+        assert len(blocks) < 12 * n / kmax * 1.5
+        assert sum(b.num_gates for b in blocks) == len(gl)
+        got = a0.copy()
+        for b in blocks:
+            assert len(b.qubits) <= kmax
+            u = b.matrix
+            assert np.max(np.abs(u @ u.conj().T - np.eye(len(u)))) < 1e-12
+            got = apply_block(got, n, b.qubits, u)
+        assert np.max(np.abs(got - want)) < 1e-12, (n, kmax)
 
 
 def test_plan_rejects_bad_input():
