@@ -357,7 +357,10 @@ def run_blocked_workload(m, n: int, depth: int) -> dict:
     for prec, name in ((m.C128, "c128"), (m.C64, "c64")):
         b = 8 if prec == m.C64 else 16
         passes = m.blocked_pass_count(g, n, prec)
-        res = {"passes": passes, "passes_per_layer": round(passes / depth, 2)}
+        m.set_blocked_lookahead(0)
+        greedy = m.blocked_pass_count(g, n, prec)   # the planner without lookahead, for comparison
+        m.set_blocked_lookahead()
+        res = {"passes": passes, "passes_per_layer": round(passes / depth, 2), "passes_greedy_planner": greedy}
         st = m.StateVector(n, prec)
         t_reg = timed(st, blocked=True)
         res["register_seconds"] = round(t_reg, 4)
