@@ -498,29 +498,38 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
     // that takes the most ops wins (the first such, in seed order).  A window
     // of neighbouring qubits lets one pass run many layers of a nearest-
     // neighbour circuit inside it.
-    std::vector<const GOp *> rest, later, best_cur, best_later, try_cur, try_later;
+    std::vector<const GOp *> rest, later;
     rest.reserve(gops.size());
     for (const GOp &o : gops) rest.push_back(&o);
-    auto scan = [&](uint64_t S0, std::vector<const GOp *> &take, std::vector<const GOp *> &defer) {
+    // ops a tile set starting from S0 takes among rest[0, lim), growing while
+    // it has free bits; with take / defer, the full split of rest
+    auto scan = [&](uint64_t S0, size_t lim, std::vector<const GOp *> *take, std::vector<const GOp *> *defer) {
         uint64_t Sx = S0, blocked = 0;
-        take.clear();
-        defer.clear();
-        for (const GOp *o : rest) {
+        size_t taken = 0;
+        for (size_t i = 0; i < lim; i++) {
+            const GOp *o = rest[i];
             const uint64_t t = touches(*o), need = needs_local(*o);
             if (!(t & blocked) && __builtin_popcountll(Sx | need) <= tb) {
                 Sx |= need;
-                take.push_back(o);
+                taken++;
+                if (take) take->push_back(o);
             } else {
                 blocked |= t;
-                defer.push_back(o);
+                if (defer) defer->push_back(o);
             }
         }
-        return Sx;
+        if (take) S = Sx;
+        return taken;
     };
+    // candidates are compared on the first kLookaheadOps pending ops, which
+    // bounds planning time on very long circuits
+    constexpr size_t kLookaheadOps = 4096;
     std::vector<uint64_t> tried;
     const int free_bits = tb - __builtin_popcountll(anchor_mask);
     while (!rest.empty()) {
-        uint64_t bestS = scan(anchor_mask, best_cur, best_later);
+        const size_t lim = rest.size() < kLookaheadOps ? rest.size() : kLookaheadOps;
+        uint64_t bestS0 = anchor_mask;
+        size_t best = lookahead > 0 ? scan(anchor_mask, lim, nullptr, nullptr) : 0;
         tried.clear();
         int seeds = 0;
         for (size_t i = 0; i < rest.size() && seeds < lookahead; i++) {
@@ -536,18 +545,17 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
                 for (int k = lo - 1; k >= 0 && __builtin_popcountll(S0) < tb; k--) S0 |= 1ull << k;
                 if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
                 tried.push_back(S0);
-                const uint64_t Sx = scan(S0, try_cur, try_later);
-                if (try_cur.size() > best_cur.size()) {
-                    bestS = Sx;
-                    best_cur.swap(try_cur);
-                    best_later.swap(try_later);
+                const size_t c = scan(S0, lim, nullptr, nullptr);
+                if (c > best) {
+                    best = c;
+                    bestS0 = S0;
                 }
             }
         }
-        S = bestS;
-        cur = best_cur;
+        later.clear();
+        scan(bestS0, rest.size(), &cur, &later);
         close();
-        rest.swap(best_later);
+        rest.swap(later);
     }
 }
 

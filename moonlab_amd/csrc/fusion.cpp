@@ -275,7 +275,7 @@ std::vector<FusedBlock> plan_fusion(const moonlab_hip_gate_t *g, int ng, int kma
 std::vector<FusedBlock> plan_fusion_windows(const moonlab_hip_gate_t *g, int ng, int kmax, int seeds, int n)
 {
     std::vector<FusedBlock> blocks;
-    std::vector<const moonlab_hip_gate_t *> rest, best_take, best_defer, take, defer;
+    std::vector<const moonlab_hip_gate_t *> rest, best_take, best_defer;
     std::vector<uint64_t> qm((size_t)ng);
     for (int i = 0; i < ng; i++) {
         int q[3];
@@ -286,27 +286,36 @@ std::vector<FusedBlock> plan_fusion_windows(const moonlab_hip_gate_t *g, int ng,
         rest.push_back(&g[i]);
     }
     auto mask = [&](const moonlab_hip_gate_t *x) { return qm[(size_t)(x - g)]; };
-    auto scan = [&](uint64_t S0, bool grow) {
+    // gates a qubit set starting from S0 takes among rest[0, lim) (growing
+    // while it has room when grow is set); with take / defer, the full split
+    auto scan = [&](uint64_t S0, bool grow, size_t lim, std::vector<const moonlab_hip_gate_t *> *tk,
+                    std::vector<const moonlab_hip_gate_t *> *df) {
         uint64_t S = S0, blocked = 0;
-        take.clear();
-        defer.clear();
-        for (const moonlab_hip_gate_t *x : rest) {
+        size_t taken = 0;
+        for (size_t i = 0; i < lim; i++) {
+            const moonlab_hip_gate_t *x = rest[i];
             const uint64_t m = mask(x);
             const bool fits = grow ? __builtin_popcountll(S | m) <= kmax : (m & ~S) == 0;
             if (!(m & blocked) && fits) {
                 S |= m;
-                take.push_back(x);
+                taken++;
+                if (tk) tk->push_back(x);
             } else {
                 blocked |= m;
-                defer.push_back(x);
+                if (df) df->push_back(x);
             }
         }
+        return taken;
     };
+    // candidates are compared on the first 4096 pending gates, which bounds
+    // planning time on very long circuits
+    constexpr size_t kLookaheadGates = 4096;
     std::vector<uint64_t> tried;
     while (!rest.empty()) {
-        scan(0, true);
-        best_take.swap(take);
-        best_defer.swap(defer);
+        const size_t lim = rest.size() < kLookaheadGates ? rest.size() : kLookaheadGates;
+        size_t best = scan(0, true, lim, nullptr, nullptr);
+        uint64_t bestS0 = 0;
+        bool best_grow = true;
         tried.clear();
         int used = 0;
         for (size_t i = 0; i < rest.size() && used < seeds; i++) {
@@ -321,13 +330,17 @@ std::vector<FusedBlock> plan_fusion_windows(const moonlab_hip_gate_t *g, int ng,
                 for (int k = lo - 1; k >= 0 && __builtin_popcountll(S0) < kmax; k--) S0 |= 1ull << k;
                 if (std::find(tried.begin(), tried.end(), S0) != tried.end()) continue;
                 tried.push_back(S0);
-                scan(S0, false);
-                if (take.size() > best_take.size()) {
-                    best_take.swap(take);
-                    best_defer.swap(defer);
+                const size_t c = scan(S0, false, lim, nullptr, nullptr);
+                if (c > best) {
+                    best = c;
+                    bestS0 = S0;
+                    best_grow = false;
                 }
             }
         }
+        best_take.clear();
+        best_defer.clear();
+        scan(bestS0, best_grow, rest.size(), &best_take, &best_defer);
         if (best_take.empty()) {   // a gate wider than kmax stands alone
             best_take.assign(rest.begin(), rest.begin() + 1);
             best_defer.assign(rest.begin() + 1, rest.end());
