@@ -234,6 +234,25 @@ MOONLAB_API int moonlab_mi355x_server_wait(moonlab_mi355x_server_t *s, uint64_t 
 MOONLAB_API void moonlab_mi355x_response_free(moonlab_mi355x_response_t *r);
 MOONLAB_API int moonlab_mi355x_server_stats(const moonlab_mi355x_server_t *s, moonlab_mi355x_server_stats_t *st);
 
+/* ---- Controlled and multi-qubit gates through the circuit API -------------
+ *
+ * Addition: records one gate of codes 14..24 of the state-vector gate table
+ * (moonlab_hip_gate_type_t: S^dagger, T^dagger, PHASE, U3, CPHASE, CRX, CRY,
+ * CRZ, true CY, Toffoli, Fredkin; gates.c:341-1048) in a circuit, which
+ * moonlab_qgtl_execute and the control-plane frames then run on the engine.
+ * qubits[] in the order of moonlab_hip_gate_t's target, control and extra
+ * fields (Toffoli: target, control, control 2; Fredkin: target, control,
+ * target 2); params: theta (PHASE, CPHASE, CRX, CRY, CRZ) or theta, phi,
+ * lambda (U3).  Returns MOONLAB_QGTL_BAD_ARG for another code, a wrong qubit
+ * or parameter count, or a repeated or out-of-range qubit.  The text format
+ * writes them as SDG, TDG, P, U3, CPHASE, CRX, CRY, CRZ, CYT, CCX and CSWAP
+ * lines (qubits, then parameters as %.17g); the reference's parser rejects
+ * those lines, so only circuits without them stay readable by it.  Gates of
+ * these codes added through moonlab_qgtl_add_gate keep the reference's
+ * behaviour: moonlab_qgtl_execute returns MOONLAB_QGTL_UNSUPPORTED. */
+MOONLAB_API int moonlab_mi355x_circuit_add_gate_ex(moonlab_qgtl_circuit_t *c, int type, const int *qubits,
+                                                   int num_qubits, const double *params, int num_params);
+
 /* ---- The control plane's wire format as the serving front door ------------
  *
  * Status codes of src/control/control_plane.h:53-62. */

@@ -249,6 +249,8 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_qgtl_results_free": ([ctypes.POINTER(_QgtlResults)], None),
         "moonlab_qgtl_circuit_num_qubits": ([vp], ip),
         "moonlab_qgtl_circuit_num_gates": ([vp], ip),
+        "moonlab_mi355x_circuit_add_gate_ex": ([vp, ip, ctypes.POINTER(ctypes.c_int), ip,
+                                                ctypes.POINTER(ctypes.c_double), ip], ip),
         "moonlab_qgtl_circuit_serialize": ([vp, ctypes.c_char_p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)], ip),
         "moonlab_qgtl_circuit_deserialize": ([ctypes.c_char_p, ctypes.c_size_t,
@@ -563,6 +565,16 @@ class Circuit:
         rc = self.add_gate_raw(gate, target, control, theta)
         if rc:
             raise QGTLError(f"add_gate: status {rc}")
+        return self
+
+    def add_gate_ex(self, gate: int, qubits, params=()) -> "Circuit":
+        """A gate of codes 14..24 (moonlab_mi355x_circuit_add_gate_ex): qubits
+        in target, control, extra order; params theta (or theta, phi, lambda)."""
+        qs = (ctypes.c_int * len(qubits))(*[int(q) for q in qubits])
+        ps = (ctypes.c_double * max(len(params), 1))(*[float(x) for x in params])
+        rc = lib().moonlab_mi355x_circuit_add_gate_ex(self._h, int(gate), qs, len(qubits), ps, len(params))
+        if rc:
+            raise QGTLError(f"add_gate_ex: status {rc}")
         return self
 
     def execute(self, num_shots: int = 0, rng_seed: int = 0,

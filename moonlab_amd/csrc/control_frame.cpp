@@ -91,7 +91,8 @@ int handle(moonlab_mi355x_server_t *srv, const char *f, size_t len, uint64_t see
     const std::vector<mlx::GateRec> &recs = mlx::qgtl_circuit_gates(c);
     std::vector<moonlab_hip_gate_t> gates(recs.size());
     for (size_t i = 0; i < recs.size(); i++)
-        gates[i] = moonlab_hip_gate_t{recs[i].type, recs[i].target, recs[i].control, -1, recs[i].theta, 0.0, 0.0};
+        gates[i] = moonlab_hip_gate_t{recs[i].type, recs[i].target, recs[i].control, recs[i].extra,
+                                      recs[i].theta,   recs[i].phi,    recs[i].lam};
     moonlab_qgtl_circuit_free(c);
     if (!srv) return reply(out, MOONLAB_CONTROL_REJECTED, "execute");
 

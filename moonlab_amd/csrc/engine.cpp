@@ -144,7 +144,7 @@ int run_circuit(const RunRequest &rq, RunResult &rr)
     std::vector<moonlab_hip_gate_t> g((size_t)rq.num_gates);
     for (int i = 0; i < rq.num_gates; i++) {
         const GateRec &r = rq.gates[i];
-        g[(size_t)i] = moonlab_hip_gate_t{r.type, r.target, r.control, -1, r.theta, 0.0, 0.0};
+        g[(size_t)i] = moonlab_hip_gate_t{r.type, r.target, r.control, r.extra, r.theta, r.phi, r.lam};
     }
     return run_circuit_gates(rq, g.data(), rq.num_gates, rr);
 }

@@ -18,6 +18,11 @@ struct GateRec {
     int    control;
     double theta;
     int    has_param;
+    // gates added through moonlab_mi355x_circuit_add_gate_ex (codes 14..24 of
+    // the state-vector gate table): second control or target, phi and lambda
+    int    extra = -1;
+    double phi = 0.0, lam = 0.0;
+    int    ext = 0;
 };
 
 bool gate_takes_param(int type);   // RX, RY, RZ

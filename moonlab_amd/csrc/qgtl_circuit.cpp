@@ -28,6 +28,24 @@ namespace {
 
 const char *kNames[14] = {"I", "X", "Y", "Z", "H", "S", "T", "RX", "RY", "RZ", "CNOT", "CY", "CZ", "SWAP"};
 
+// Extension gates (moonlab_mi355x_circuit_add_gate_ex): codes 14..24 of the
+// state-vector gate table, their text names, qubit count and parameters.
+struct ExtGate {
+    const char *name;
+    int         nq, np;
+};
+const ExtGate kExt[11] = {{"SDG", 1, 0},    {"TDG", 1, 0},    {"P", 1, 1},    {"U3", 1, 3},
+                          {"CPHASE", 2, 1}, {"CRX", 2, 1},    {"CRY", 2, 1},  {"CRZ", 2, 1},
+                          {"CYT", 2, 0},    {"CCX", 3, 0},    {"CSWAP", 3, 0}};
+constexpr int kExtFirst = MOONLAB_HIP_GATE_SDG;
+
+int ext_from_name(const char *s)
+{
+    for (int i = 0; i < 11; i++)
+        if (strcmp(s, kExt[i].name) == 0) return kExtFirst + i;
+    return -1;
+}
+
 int gate_from_name(const char *s, int *out)
 {
     for (int i = 0; i < 14; i++)
@@ -100,12 +118,37 @@ int moonlab_qgtl_add_gate(moonlab_qgtl_circuit_t *c, moonlab_qgtl_gate_t type, i
     return MOONLAB_QGTL_OK;
 }
 
+int moonlab_mi355x_circuit_add_gate_ex(moonlab_qgtl_circuit_t *c, int type, const int *qubits, int num_qubits,
+                                       const double *params, int num_params)
+{
+    if (!c || type < kExtFirst || type > MOONLAB_HIP_GATE_FREDKIN) return MOONLAB_QGTL_BAD_ARG;
+    const ExtGate &e = kExt[type - kExtFirst];
+    if (num_qubits != e.nq || !qubits || num_params != e.np || (e.np && !params)) return MOONLAB_QGTL_BAD_ARG;
+    for (int j = 0; j < e.nq; j++) {
+        if (qubits[j] < 0 || qubits[j] >= c->num_qubits) return MOONLAB_QGTL_BAD_ARG;
+        for (int k = 0; k < j; k++)
+            if (qubits[k] == qubits[j]) return MOONLAB_QGTL_BAD_ARG;
+    }
+    GateRec g{type, qubits[0], e.nq > 1 ? qubits[1] : -1, e.np ? params[0] : 0.0, e.np ? 1 : 0};
+    g.extra = e.nq > 2 ? qubits[2] : -1;
+    g.phi = e.np > 1 ? params[1] : 0.0;
+    g.lam = e.np > 2 ? params[2] : 0.0;
+    g.ext = 1;
+    try {
+        c->gates.push_back(g);
+    } catch (...) {
+        return MOONLAB_QGTL_OOM;
+    }
+    return MOONLAB_QGTL_OK;
+}
+
 int moonlab_qgtl_execute(moonlab_qgtl_circuit_t *c, const moonlab_qgtl_exec_options_t *opts,
                          moonlab_qgtl_results_t *out)
 {
     if (!c || !opts || !out) return MOONLAB_QGTL_BAD_ARG;
+    // codes outside the reference's 14 run only when added as extensions
     for (const GateRec &g : c->gates)
-        if (g.type < 0 || g.type > MOONLAB_QGTL_GATE_SWAP) return MOONLAB_QGTL_UNSUPPORTED;
+        if (g.type < 0 || (g.type > MOONLAB_QGTL_GATE_SWAP && !g.ext)) return MOONLAB_QGTL_UNSUPPORTED;
     std::vector<double> u;
     if (opts->num_shots > 0) {
         uint64_t seed = opts->rng_seed ? opts->rng_seed : ((uint64_t)time(NULL) ^ 0x9e3779b97f4a7c15ULL);
@@ -182,6 +225,18 @@ int moonlab_qgtl_circuit_serialize(const moonlab_qgtl_circuit_t *c, char *buf, s
     if (n < 0) return MOONLAB_QGTL_INTERNAL;
     if (!emit(n)) return MOONLAB_QGTL_OOM;
     for (const GateRec &g : c->gates) {
+        if (g.ext) {   // extension gates: NAME qubits... params...
+            const ExtGate &e = kExt[g.type - kExtFirst];
+            const int qs[3] = {g.target, g.control, g.extra};
+            const double ps[3] = {g.theta, g.phi, g.lam};
+            n = snprintf(tmp, sizeof tmp, "%s", e.name);
+            for (int j = 0; j < e.nq && n > 0; j++) n += snprintf(tmp + n, sizeof tmp - (size_t)n, " %d", qs[j]);
+            for (int j = 0; j < e.np && n > 0; j++) n += snprintf(tmp + n, sizeof tmp - (size_t)n, " %.17g", ps[j]);
+            if (n > 0) n += snprintf(tmp + n, sizeof tmp - (size_t)n, "\n");
+            if (n <= 0 || (size_t)n >= sizeof tmp) return MOONLAB_QGTL_INTERNAL;
+            if (!emit(n)) return MOONLAB_QGTL_OOM;
+            continue;
+        }
         if (g.type < 0 || g.type > 13) return MOONLAB_QGTL_INTERNAL;
         const char *name = kNames[g.type];
         if (mlx::gate_is_two_qubit(g.type)) n = snprintf(tmp, sizeof tmp, "%s %d %d\n", name, g.target, g.control);
@@ -229,8 +284,31 @@ moonlab_qgtl_circuit_t *moonlab_qgtl_circuit_deserialize(const char *buf, size_t
         int consumed = 0, type = 0, target = -1, control = -1, rc;
         double theta = 0.0;
         if (sscanf(line, "%7s %n", name, &consumed) != 1) { status = MOONLAB_QGTL_BAD_ARG; goto done; }
-        if (gate_from_name(name, &type) != MOONLAB_QGTL_OK) { status = MOONLAB_QGTL_BAD_ARG; goto done; }
         const char *rest = line + consumed;
+        if (gate_from_name(name, &type) != MOONLAB_QGTL_OK) {
+            // an extension gate: its qubits, then its parameters
+            const int xt = ext_from_name(name);
+            if (xt < 0) { status = MOONLAB_QGTL_BAD_ARG; goto done; }
+            const ExtGate &e = kExt[xt - kExtFirst];
+            int qs[3] = {-1, -1, -1};
+            double ps[3] = {0.0, 0.0, 0.0};
+            const char *p = rest;
+            bool ok = true;
+            for (int j = 0; j < e.nq && ok; j++) {
+                int used = 0;
+                ok = sscanf(p, "%d%n", &qs[j], &used) == 1;
+                p += used;
+            }
+            for (int j = 0; j < e.np && ok; j++) {
+                int used = 0;
+                ok = sscanf(p, "%lf%n", &ps[j], &used) == 1;
+                p += used;
+            }
+            if (!ok) { status = MOONLAB_QGTL_BAD_ARG; goto done; }
+            rc = moonlab_mi355x_circuit_add_gate_ex(c, xt, qs, e.nq, ps, e.np);
+            if (rc != MOONLAB_QGTL_OK) { status = rc; goto done; }
+            continue;
+        }
         if (mlx::gate_is_two_qubit(type)) {
             if (sscanf(rest, "%d %d", &target, &control) != 2) { status = MOONLAB_QGTL_BAD_ARG; goto done; }
             rc = moonlab_qgtl_add_gate(c, (moonlab_qgtl_gate_t)type, target, control, nullptr);

@@ -362,3 +362,33 @@ def test_control_frame_parsers_under_sanitizers(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("fuzzed 200000 frames"), r.stdout
+
+
+def test_circuit_extension_gates_text_and_validation():
+    """moonlab_mi355x_circuit_add_gate_ex: codes 14..24 recorded with their
+    qubits and parameters, written and read back as extension lines; v1
+    gates keep their lines; bad arity, repeats and codes are refused; a code
+    above 13 added through moonlab_qgtl_add_gate still executes as the
+    reference's UNSUPPORTED (checked before any device work)."""
+    c = m.Circuit(4).add_gate(m.GateType.H, 0)
+    c.add_gate_ex(14, [1]).add_gate_ex(16, [2], [0.25]).add_gate_ex(17, [3], [0.1, 0.2, 0.3])
+    c.add_gate_ex(18, [0, 1], [1.5]).add_gate_ex(21, [2, 3], [-0.75]).add_gate_ex(22, [1, 0])
+    c.add_gate_ex(23, [3, 0, 1]).add_gate_ex(24, [2, 0, 3])
+    text = c.serialize()
+    lines = text.splitlines()
+    assert lines[2] == "H 0" and lines[3] == "SDG 1" and lines[4] == "P 2 0.25"
+    assert lines[5] == "U3 3 0.10000000000000001 0.20000000000000001 0.29999999999999999"
+    assert lines[6] == "CPHASE 0 1 1.5" and lines[9] == "CCX 3 0 1" and lines[10] == "CSWAP 2 0 3"
+    c2 = m.Circuit.deserialize(text)
+    assert c2.serialize() == text and c2.num_gates == 9
+    for bad in [(13, [0], []), (25, [0], []), (14, [0, 1], []), (16, [0], []), (17, [0], [1.0]),
+                (23, [0, 0, 1], []), (18, [0, 4], [1.0])]:
+        with pytest.raises(m.QGTLError):
+            m.Circuit(4).add_gate_ex(*bad)
+    for bad in ["NUM_QUBITS 2\nCCX 0 1\n", "NUM_QUBITS 2\nP 0\n", "NUM_QUBITS 3\nCSWAP 0 1 1\n"]:
+        with pytest.raises(m.QGTLError):
+            m.Circuit.deserialize(bad)
+    raw = m.Circuit(2)
+    assert raw.add_gate_raw(14, 0) == 0          # recorded, as the reference records it
+    with pytest.raises(m.QGTLError, match="-303"):
+        raw.execute(return_probabilities=True)

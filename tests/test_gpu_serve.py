@@ -152,3 +152,42 @@ def test_control_frames_match_reference_vectors(eng, oracle):
         assert np.max(np.abs(p - oracle.probabilities(n, a))) < 1e-13
         _, o, _ = _decode(alone[2 * i + 1][1], np.uint64)
         assert o.tolist() == oracle.qgtl_shots(n, a, 300, 1000 + i).tolist()
+
+
+def test_extension_gates_through_circuit_api_and_frames(eng, oracle):
+    """Controlled and multi-qubit gates added with add_gate_ex run through
+    moonlab_qgtl_execute and through a CIRCUIT frame: probabilities against
+    the oracle's gate table (1e-13); seeded SHOTS frames equal the circuit
+    API's outcomes for the same seed."""
+    rng = np.random.default_rng(12)
+    n = 9
+    c = eng.Circuit(n)
+    gl = []
+    for _ in range(80):
+        t = int(rng.integers(0, 25))
+        qs = [int(x) for x in rng.permutation(n)[:3]]
+        if t <= 13:
+            th = float(rng.uniform(-3, 3)) if t in (7, 8, 9) else None
+            ctl = qs[1] if t >= 10 else -1
+            c.add_gate(t, qs[0], ctl, th)
+            gl.append((t, qs[0], ctl, -1, th or 0.0))
+        else:
+            nq = 3 if t in (23, 24) else (2 if t >= 18 else 1)
+            npar = 3 if t == 17 else (1 if t in (16, 18, 19, 20, 21) else 0)
+            ps = [float(x) for x in rng.uniform(-3, 3, npar)]
+            c.add_gate_ex(t, qs[:nq], ps)
+            full = qs[:nq] + [-1] * (3 - nq)
+            pp = ps + [0.0] * (3 - npar)
+            gl.append((t, full[0], full[1], full[2], pp[0], pp[1], pp[2]))
+    want = oracle.probabilities(n, oracle.run(n, gl))
+    r = c.execute(num_shots=64, rng_seed=99, return_probabilities=True)
+    assert np.max(np.abs(r.probabilities - want)) < 1e-13
+    text = c.serialize().encode()
+    with eng.Server() as srv:
+        rc, resp = eng.handle_frame(b"CIRCUIT %d\n" % len(text) + text, srv)
+        assert rc == eng.CONTROL_OK
+        p = np.frombuffer(resp.partition(b"\n")[2], dtype=np.float64)
+        assert np.max(np.abs(p - want)) < 1e-13
+        rc, resp = eng.handle_frame(b"SHOTS 64 %d\n" % len(text) + text, srv, seed=99)
+        assert rc == eng.CONTROL_OK
+        assert np.frombuffer(resp.partition(b"\n")[2], dtype=np.uint64).tolist() == r.outcomes.tolist()
