@@ -286,6 +286,17 @@ MOONLAB_API int moonlab_mi355x_circuit_add_gate_ex(moonlab_qgtl_circuit_t *c, in
 MOONLAB_API int moonlab_mi355x_server_handle_frame(moonlab_mi355x_server_t *srv, const void *frame, size_t len,
                                                    uint64_t seed, void **out_response, size_t *out_len);
 
+/* The same with the connection's tenant (control_plane.c:699-731; NULL or ""
+ * for none): while the request runs the tenant is the calling thread's
+ * request context (moonlab_scheduler_set_request_context), and after a
+ * completed CIRCUIT or SHOTS request the completion hook fires with
+ * job = NULL, backend "control_plane", num_qubits and total_shots (0 for
+ * CIRCUIT); the context is cleared before returning.
+ * moonlab_mi355x_server_handle_frame is this call with tenant_id = NULL. */
+MOONLAB_API int moonlab_mi355x_server_handle_frame_ex(moonlab_mi355x_server_t *srv, const void *frame, size_t len,
+                                                      uint64_t seed, const char *tenant_id, void **out_response,
+                                                      size_t *out_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -279,6 +279,9 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_mi355x_server_handle_frame": ([vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint64,
                                                 ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t)],
                                                ip),
+        "moonlab_mi355x_server_handle_frame_ex": ([vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint64,
+                                                   ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
+                                                   ctypes.POINTER(ctypes.c_size_t)], ip),
         "moonlab_mi355x_hea_tfim_energies": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
         "moonlab_mi355x_hea_tfim_gradients": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, dp,
                                                ctypes.POINTER(_BatchStats)], ip),
@@ -1060,14 +1063,22 @@ _libc.free.argtypes = [ctypes.c_void_p]
 _libc.free.restype = None
 
 
-def handle_frame(frame: bytes, server: Optional["Server"] = None, seed: int = 0) -> Tuple[int, bytes]:
+def handle_frame(frame: bytes, server: Optional["Server"] = None, seed: int = 0,
+                 tenant: Optional[str] = None) -> Tuple[int, bytes]:
     """One control-plane request frame (CIRCUIT / SHOTS / HEALTH) through
-    moonlab_mi355x_server_handle_frame: returns (status, response frame).
-    Without a server only HEALTH and frames the parser rejects are answered."""
+    moonlab_mi355x_server_handle_frame (with a tenant: _ex, which sets the
+    request context and fires the completion hook): returns (status,
+    response frame).  Without a server only HEALTH and frames the parser
+    rejects are answered."""
     out = ctypes.c_void_p()
     n = ctypes.c_size_t(0)
-    rc = lib().moonlab_mi355x_server_handle_frame(server._h if server is not None else None, frame, len(frame),
-                                                  seed, ctypes.byref(out), ctypes.byref(n))
+    h = server._h if server is not None else None
+    if tenant is None:
+        rc = lib().moonlab_mi355x_server_handle_frame(h, frame, len(frame), seed, ctypes.byref(out),
+                                                      ctypes.byref(n))
+    else:
+        rc = lib().moonlab_mi355x_server_handle_frame_ex(h, frame, len(frame), seed, tenant.encode(),
+                                                         ctypes.byref(out), ctypes.byref(n))
     try:
         data = ctypes.string_at(out.value, n.value) if out.value else b""
     finally:

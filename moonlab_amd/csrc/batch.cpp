@@ -426,6 +426,20 @@ uint64_t touches(const GOp &o)
 
 }  // namespace
 
+bool segments_fit_staging(const BOp *ops, int count)
+{
+    for (int k = 0; k < count;) {
+        if (ops[k].kind == BOP_SEG) {
+            const int lq = ops[k].lq;
+            if (lq < 1 || 1 + lq > kOpBuf || k + 1 + lq > count) return false;
+            k += 1 + lq;
+        } else {
+            k++;
+        }
+    }
+    return true;
+}
+
 // Step 3: pack ops into passes and convert qubits to tile-local indices.
 void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std::vector<BPass> &passes,
                  std::vector<BOp> &ops, int anchor_bits, int amp_bytes, int reg_bits, int lookahead)
@@ -691,6 +705,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         std::vector<BPass> passes;
         std::vector<BOp>   ops;
         bool               bad = false;
+        bool               internal = false;
     };
     const int nthreads = host_threads(count);
     std::vector<Chunk> chunks((size_t)nthreads);
@@ -717,6 +732,13 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
             I.first_pass = (int)c.passes.size();   // chunk-local (passes and ops) until placed
             plan_passes(gops, n, I.tile_bits, kSegRegsBatch, c.passes, c.ops);
             I.num_passes = (int)c.passes.size() - I.first_pass;
+            for (int p = I.first_pass; p < I.first_pass + I.num_passes; p++) {
+                const BPass &P = c.passes[(size_t)p];
+                if (!segments_fit_staging(c.ops.data() + P.first_op, P.num_ops)) {
+                    c.internal = true;   // a planner bug: the staged op loop could not advance
+                    return;
+                }
+            }
             // longest first: sweeps x amplitudes (resident circuits: ops x tile)
             const double sweeps = n > I.tile_bits ? (double)(I.num_passes + 2) * 4.0 : 0.2 * (double)gops.size();
             cost[(size_t)i] = {-(sweeps * (double)(1ull << n)), i};
@@ -734,6 +756,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     for (int t = 0; t < nthreads; t++) {
         const Chunk &c = chunks[(size_t)t];
         if (c.bad) return MOONLAB_SCHED_BAD_ARG;
+        if (c.internal) return MOONLAB_SCHED_INTERNAL;
         pass_base[(size_t)t] = total_passes;
         op_base[(size_t)t] = total_ops;
         total_passes += c.passes.size();

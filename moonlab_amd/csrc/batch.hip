@@ -300,7 +300,11 @@ __device__ void run_pass_ops_staged(Shared &sh, cplx_t<R> *t, int T, const BOp *
             }
             __syncthreads();
         }
-        k += j;   // every thread has passed the last barrier: opbuf may be overwritten
+        // every thread has passed the last barrier: opbuf may be overwritten.
+        // The host refuses plans whose segments do not fit one chunk
+        // (segments_fit_staging), so j > 0 here; should one ever reach the
+        // device, its segment is skipped rather than restaged forever.
+        k += j ? j : 1 + sh.opbuf[0].lq;
     }
 }
 

@@ -40,4 +40,12 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
                  std::vector<BOp> &ops, int anchor_bits = 4, int amp_bytes = 16, int reg_bits = kRegBits,
                  int lookahead = 0);
 
+// The batch kernel stages a pass's op descriptors in LDS kOpBuf at a time
+// and applies a register segment only when its header and all of its ops sit
+// in one staged chunk (run_pass_ops_staged, batch.hip).  True when every
+// segment of ops[first, first + count) fits: header + ops <= kOpBuf and no
+// segment runs past the pass's op range.  A plan that fails this would make
+// the staged loop refuse to advance.
+bool segments_fit_staging(const BOp *ops, int count);
+
 }  // namespace mlx

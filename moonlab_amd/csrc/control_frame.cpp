@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/moonlab_mi355x.h"
+#include "../../include/moonlab_scheduler.h"
 #include "engine.h"
 
 namespace mlx {
@@ -54,7 +55,8 @@ void append_binary(std::string &out, const T *v, size_t n)
     out.append((const char *)v, n * sizeof(T));
 }
 
-int handle(moonlab_mi355x_server_t *srv, const char *f, size_t len, uint64_t seed, std::string &out)
+int handle(moonlab_mi355x_server_t *srv, const char *f, size_t len, uint64_t seed, const char *tenant,
+           std::string &out)
 {
     const char *nl = (const char *)memchr(f, '\n', len < kVerbLine ? len : kVerbLine);
     if (!nl) return reply(out, MOONLAB_CONTROL_PROTOCOL, "no verb line");   // EOF before '\n', control_plane.c:176
@@ -104,15 +106,30 @@ int handle(moonlab_mi355x_server_t *srv, const char *f, size_t len, uint64_t see
     q.num_shots = shots_mode ? (int)num_shots : 0;
     q.rng_seed = seed;                 // 0: clock-based, as control_plane.c:662-664
     q.want_probabilities = shots_mode ? 0 : 1;
+    // Tenant attribution as control_plane.c:699-731: the tenant (NULL when
+    // the connection has none) is the calling thread's request context while
+    // the request runs, and a completed run fires the completion hook with
+    // the fields billing / audit hooks read (num_qubits, total_shots).
+    moonlab_scheduler_set_request_context(tenant && *tenant ? tenant : nullptr, nullptr);
     uint64_t ticket = 0;
-    if (moonlab_mi355x_server_submit(srv, &q, &ticket) != MOONLAB_SCHED_OK)
+    if (moonlab_mi355x_server_submit(srv, &q, &ticket) != MOONLAB_SCHED_OK) {
+        moonlab_scheduler_set_request_context(nullptr, nullptr);
         return reply(out, MOONLAB_CONTROL_REJECTED, "execute");
+    }
     moonlab_mi355x_response_t r;
     const int wr = moonlab_mi355x_server_wait(srv, ticket, &r, -1.0);
     if (wr != MOONLAB_SCHED_OK || r.status != MOONLAB_SCHED_OK) {
         moonlab_mi355x_response_free(&r);
+        moonlab_scheduler_set_request_context(nullptr, nullptr);
         return reply(out, MOONLAB_CONTROL_REJECTED, "execute");
     }
+    {
+        moonlab_job_results_t hook_res{};
+        hook_res.num_qubits = n;
+        hook_res.total_shots = shots_mode ? r.num_shots : 0;
+        moonlab_scheduler_fire_completion_hook(nullptr, &hook_res, "control_plane");
+    }
+    moonlab_scheduler_set_request_context(nullptr, nullptr);
     char hdr[64];
     if (shots_mode) {
         snprintf(hdr, sizeof hdr, "SAMPLES %d\n", r.num_shots);
@@ -130,14 +147,15 @@ int handle(moonlab_mi355x_server_t *srv, const char *f, size_t len, uint64_t see
 
 }  // namespace
 
-extern "C" int moonlab_mi355x_server_handle_frame(moonlab_mi355x_server_t *srv, const void *frame, size_t len,
-                                                  uint64_t seed, void **out_response, size_t *out_len)
+extern "C" int moonlab_mi355x_server_handle_frame_ex(moonlab_mi355x_server_t *srv, const void *frame, size_t len,
+                                                     uint64_t seed, const char *tenant_id, void **out_response,
+                                                     size_t *out_len)
 {
     if (!out_response || !out_len) return MOONLAB_CONTROL_BAD_ARG;
     *out_response = nullptr;
     *out_len = 0;
     std::string out;
-    const int rc = frame ? handle(srv, (const char *)frame, len, seed, out)
+    const int rc = frame ? handle(srv, (const char *)frame, len, seed, tenant_id, out)
                          : reply(out, MOONLAB_CONTROL_BAD_ARG, "no frame");
     void *buf = malloc(out.size());
     if (!buf) return MOONLAB_CONTROL_OOM;
@@ -145,4 +163,10 @@ extern "C" int moonlab_mi355x_server_handle_frame(moonlab_mi355x_server_t *srv, 
     *out_response = buf;
     *out_len = out.size();
     return rc;
+}
+
+extern "C" int moonlab_mi355x_server_handle_frame(moonlab_mi355x_server_t *srv, const void *frame, size_t len,
+                                                  uint64_t seed, void **out_response, size_t *out_len)
+{
+    return moonlab_mi355x_server_handle_frame_ex(srv, frame, len, seed, nullptr, out_response, out_len);
 }

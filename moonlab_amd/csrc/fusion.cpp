@@ -138,6 +138,16 @@ int gate_qubits(const moonlab_hip_gate_t &g, int q[3])
     return n;
 }
 
+bool gate_valid(const moonlab_hip_gate_t &g, int n)
+{
+    if (g.type < 0 || g.type > MOONLAB_HIP_GATE_FREDKIN) return false;
+    int q[3];
+    const int k = gate_qubits(g, q);
+    for (int j = 0; j < k; j++)
+        if (q[j] < 0 || q[j] >= n || (j && q[j] == q[j - 1])) return false;
+    return true;
+}
+
 void gate_dense(const moonlab_hip_gate_t &g, cd *m)
 {
     int q[3];

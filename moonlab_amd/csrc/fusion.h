@@ -24,6 +24,10 @@ struct FusedBlock {
 // Qubits a gate acts on (1..3); returns the count.
 int gate_qubits(const moonlab_hip_gate_t &g, int q[3]);
 
+// A gate every engine path can run on n qubits: a known code (0..24) and
+// distinct qubits in range.
+bool gate_valid(const moonlab_hip_gate_t &g, int n);
+
 // Dense matrix of a gate on its own qubits (local bit j = q[j] from
 // gate_qubits), row-major 2^nq x 2^nq.
 void gate_dense(const moonlab_hip_gate_t &g, cd *m);

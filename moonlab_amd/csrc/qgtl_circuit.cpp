@@ -11,6 +11,7 @@
 #include <time.h>
 
 #include <new>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -129,6 +130,10 @@ int moonlab_mi355x_circuit_add_gate_ex(moonlab_qgtl_circuit_t *c, int type, cons
         for (int k = 0; k < j; k++)
             if (qubits[k] == qubits[j]) return MOONLAB_QGTL_BAD_ARG;
     }
+    // the extension grammar is this library's own: angles must be finite
+    // (a NaN angle would answer OK with an all-NaN state)
+    for (int j = 0; j < e.np; j++)
+        if (!std::isfinite(params[j])) return MOONLAB_QGTL_BAD_ARG;
     GateRec g{type, qubits[0], e.nq > 1 ? qubits[1] : -1, e.np ? params[0] : 0.0, e.np ? 1 : 0};
     g.extra = e.nq > 2 ? qubits[2] : -1;
     g.phi = e.np > 1 ? params[1] : 0.0;

@@ -38,6 +38,7 @@
 
 #include "../../include/moonlab_mi355x.h"
 #include "engine.h"
+#include "fusion.h"
 
 namespace {
 
@@ -306,6 +307,11 @@ int moonlab_mi355x_server_submit(moonlab_mi355x_server_t *s, const moonlab_mi355
     if (q->num_qubits < 1 || q->num_qubits > cap || q->num_gates < 0 || (q->num_gates > 0 && !q->gates) ||
         q->num_shots < 0 || (q->precision != MOONLAB_HIP_C128 && q->precision != MOONLAB_HIP_C64))
         return MOONLAB_SCHED_BAD_ARG;
+    // A co-batch launch reports one status for all of its requests, so a
+    // gate the engine would reject must fail its own submit, never the batch
+    // of other tenants it would have joined.
+    for (int i = 0; i < q->num_gates; i++)
+        if (!mlx::gate_valid(q->gates[i], q->num_qubits)) return MOONLAB_SCHED_BAD_ARG;
     Req *r = new (std::nothrow) Req();
     if (!r) return MOONLAB_SCHED_OOM;
     r->n = q->num_qubits;

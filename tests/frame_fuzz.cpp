@@ -10,6 +10,7 @@
 #include <random>
 #include <vector>
 #include "moonlab_mi355x.h"
+#include "moonlab_scheduler.h"
 #include "engine.h"
 namespace mlx {
 bool gate_takes_param(int t) { return t == 7 || t == 8 || t == 9; }
@@ -23,13 +24,16 @@ int moonlab_mi355x_server_submit(moonlab_mi355x_server_t *, const moonlab_mi355x
 int moonlab_mi355x_server_wait(moonlab_mi355x_server_t *, uint64_t, moonlab_mi355x_response_t *, double) { abort(); }
 void moonlab_mi355x_response_free(moonlab_mi355x_response_t *) { abort(); }
 const char *moonlab_hip_last_error(void) { return ""; }
+void moonlab_scheduler_set_request_context(const char *, const char *) { abort(); }
+void moonlab_scheduler_fire_completion_hook(const moonlab_job_t *, const moonlab_job_results_t *, const char *) { abort(); }
 }
 int main() {
     std::mt19937_64 rng(7);
-    const char *seeds[] = {"HEALTH\n", "CIRCUIT 40\nNUM_QUBITS 3\nH 0\nCNOT 1 0\nRZ 2 0.25\n#\n", "SHOTS 16 28\nNUM_QUBITS 2\nH 0\nCNOT 1 0\n\n"};
+    const char *seeds[] = {"HEALTH\n", "CIRCUIT 40\nNUM_QUBITS 3\nH 0\nCNOT 1 0\nRZ 2 0.25\n#\n", "SHOTS 16 28\nNUM_QUBITS 2\nH 0\nCNOT 1 0\n\n",
+                           "CIRCUIT 49\nNUM_QUBITS 3\nP 0 nan\nU3 1 0.5 inf 0.25\nCCX 2 0 1\n"};
     long n = 0;
     for (int i = 0; i < 200000; i++) {
-        std::string f = seeds[i % 3];
+        std::string f = seeds[i % 4];
         int k = rng() % 3;
         if (k == 0) { int c = 1 + rng() % 5; for (int j = 0; j < c && !f.empty(); j++) f[rng() % f.size()] = (char)(rng() % 256); }
         else if (k == 1) f.resize(rng() % (f.size() + 1));

@@ -185,6 +185,8 @@ int main() {
             std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
             std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
             mlx::plan_passes(gops, n, tbb, sr, passes, ops, 4, 16, mlx::kRegBits, (trial & 1) ? mlx::kBlockedLookahead : 0);
+            for (const mlx::BPass &P : passes)
+                if (!mlx::segments_fit_staging(&ops[P.first_op], P.num_ops)) { printf("segment exceeds the op staging buffer\n"); return 1; }
             std::vector<cd> st(1ull << n, cd(0)); st[0] = 1.0;
             for (const mlx::BPass &P : passes) {
                 int T = 1 << P.nbits; uint64_t ntiles = 1ull << (n - P.nbits);
@@ -205,6 +207,33 @@ int main() {
             if (err > 1e-12) { printf("MISMATCH n=%d tb=%d sr=%d err=%g\n", n, tbb, sr, err); return 1; }
             checked++;
         }
+    }
+    // the staging bound: a long run of non-commuting ops on two qubits (RX,
+    // CZ, RY, CZ, ...) fills register segments up to kOpBuf - 1 ops, never
+    // more; a segment one op longer, or one that runs past its pass, is refused
+    {
+        const int n = 6;
+        std::vector<moonlab_hip_gate_t> g;
+        for (int k = 0; k < 200; k++) {
+            g.push_back({7 + k % 2, k % 2, -1, -1, 0.1 + 0.01 * k, 0, 0});
+            g.push_back({12, 1, 0, -1, 0, 0, 0});
+        }
+        std::vector<mlx::GOp> gops; if (!mlx::lower(g.data(), (int)g.size(), n, gops)) { printf("lower failed\n"); return 1; }
+        std::vector<mlx::BPass> passes; std::vector<mlx::BOp> ops;
+        mlx::plan_passes(gops, n, n, mlx::kSegRegsBatch, passes, ops);
+        int longest = 0;
+        for (const mlx::BPass &P : passes) {
+            if (!mlx::segments_fit_staging(&ops[P.first_op], P.num_ops)) { printf("long run: segment exceeds the op staging buffer\n"); return 1; }
+            for (int k = 0; k < P.num_ops; k++) if (ops[P.first_op + k].kind == mlx::BOP_SEG) longest = std::max(longest, (int)ops[P.first_op + k].lq);
+        }
+        if (longest != mlx::kOpBuf - 1) { printf("longest segment %d ops, expected %d\n", longest, mlx::kOpBuf - 1); return 1; }
+        std::vector<mlx::BOp> bad(mlx::kOpBuf + 1);
+        for (auto &b : bad) b.kind = mlx::BOP_MAT1;
+        bad[0].kind = mlx::BOP_SEG; bad[0].lq = mlx::kOpBuf;
+        if (mlx::segments_fit_staging(bad.data(), (int)bad.size())) { printf("oversized segment accepted\n"); return 1; }
+        bad[0].lq = mlx::kOpBuf - 1;
+        if (!mlx::segments_fit_staging(bad.data(), (int)bad.size())) { printf("largest segment refused\n"); return 1; }
+        if (mlx::segments_fit_staging(bad.data(), mlx::kOpBuf - 1)) { printf("segment past its pass accepted\n"); return 1; }
     }
     // register-phase programs: 12..15 qubits, tiles of 12 and 13, both
     // amplitude widths of the LDS swizzle

@@ -326,7 +326,10 @@ def test_control_frame_fuzz_without_device():
     way; a frame it would execute is refused here for want of a server)."""
     rng = np.random.default_rng(20261202)
     text = m.Circuit(3).add_gate(m.GateType.H, 0).add_gate(m.GateType.CNOT, 1, 0).serialize().encode()
+    ext = b"# moonlab-circuit v1\nNUM_QUBITS 2\nP 0 nan\nU3 1 0.5 inf 0.25\n"
     seeds = [b"HEALTH\n", b"CIRCUIT %d\n" % len(text) + text, b"SHOTS 16 %d\n" % len(text) + text]
+    # non-finite extension angles are refused at deserialize, before execution
+    assert m.handle_frame(b"CIRCUIT %d\n" % len(ext) + ext) == (m.CONTROL_REJECTED, b"ERR -405 deserialize\n")
     codes = {m.CONTROL_OK, m.CONTROL_BAD_ARG, m.CONTROL_PROTOCOL, m.CONTROL_REJECTED}
     for i in range(3000):
         base = bytearray(seeds[i % 3])
@@ -382,10 +385,12 @@ def test_circuit_extension_gates_text_and_validation():
     c2 = m.Circuit.deserialize(text)
     assert c2.serialize() == text and c2.num_gates == 9
     for bad in [(13, [0], []), (25, [0], []), (14, [0, 1], []), (16, [0], []), (17, [0], [1.0]),
-                (23, [0, 0, 1], []), (18, [0, 4], [1.0])]:
+                (23, [0, 0, 1], []), (18, [0, 4], [1.0]), (16, [0], [float("nan")]),
+                (17, [0], [0.1, float("inf"), 0.3]), (21, [0, 1], [float("-inf")])]:
         with pytest.raises(m.QGTLError):
             m.Circuit(4).add_gate_ex(*bad)
-    for bad in ["NUM_QUBITS 2\nCCX 0 1\n", "NUM_QUBITS 2\nP 0\n", "NUM_QUBITS 3\nCSWAP 0 1 1\n"]:
+    for bad in ["NUM_QUBITS 2\nCCX 0 1\n", "NUM_QUBITS 2\nP 0\n", "NUM_QUBITS 3\nCSWAP 0 1 1\n",
+                "NUM_QUBITS 2\nP 0 nan\n", "NUM_QUBITS 2\nU3 1 0.5 inf 0.1\n", "NUM_QUBITS 2\nCRZ 0 1 -inf\n"]:
         with pytest.raises(m.QGTLError):
             m.Circuit.deserialize(bad)
     raw = m.Circuit(2)

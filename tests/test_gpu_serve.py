@@ -191,3 +191,64 @@ def test_extension_gates_through_circuit_api_and_frames(eng, oracle):
         rc, resp = eng.handle_frame(b"SHOTS 64 %d\n" % len(text) + text, srv, seed=99)
         assert rc == eng.CONTROL_OK
         assert np.frombuffer(resp.partition(b"\n")[2], dtype=np.uint64).tolist() == r.outcomes.tolist()
+
+
+def test_bad_request_fails_alone(eng, oracle):
+    """A request with a gate the engine rejects (qubit out of range, repeated
+    qubit, unknown code) fails its own submit; the valid requests submitted
+    around it, which it would have been co-batched with, all complete and
+    match the oracle."""
+    reqs = workloads.serving_requests(12, 23, 1000.0, n_range=(6, 10), depth_range=(3, 6), shots=64)
+    bad_gates = [[(4, 0), (10, 12, 3)],          # CNOT target past n = 10
+                 [(4, 0), (10, 2, 2)],           # control == target
+                 [(4, 0), (25, 0)],              # unknown code
+                 [(4, 0), (23, 1, 0, 1)]]        # Toffoli with a repeated qubit
+    with eng.Server(batch_max_qubits=12) as srv:
+        tickets = []
+        for i, r in enumerate(reqs):
+            tickets.append(srv.submit(r["n"], r["gates"], shots=r["shots"], seed=r["shot_seed"], precision=eng.C128))
+            if i < len(bad_gates):
+                with pytest.raises(eng.SchedulerError):
+                    srv.submit(10, bad_gates[i], shots=8, seed=1, precision=eng.C128)
+        res = [srv.wait(t) for t in tickets]
+    for r, x in zip(reqs, res):
+        assert x.status == 0
+        a = oracle.run(r["n"], r["gates"])
+        assert np.max(np.abs(x.expect_z - oracle.expect_z_all(r["n"], a))) < 1e-12
+        assert x.outcomes.tolist() == oracle.qgtl_shots(r["n"], a, r["shots"], r["shot_seed"]).tolist()
+
+
+def test_frames_attribute_tenant_and_fire_hook(eng):
+    """moonlab_mi355x_server_handle_frame_ex (control_plane.c:699-731): the
+    tenant is the request context while the frame runs, the completion hook
+    fires once per completed CIRCUIT / SHOTS frame with job NULL, backend
+    "control_plane", num_qubits and total_shots (0 for CIRCUIT), never for a
+    rejected frame or HEALTH, and the context is cleared afterwards."""
+    L = eng.lib()
+    seen = []
+
+    @eng.COMPLETION_HOOK_FN
+    def hook(job, res, backend, ctx):
+        seen.append((bool(job), backend.decode(), res.contents.num_qubits, res.contents.total_shots,
+                     L.moonlab_scheduler_current_tenant_id()))
+
+    text = eng.Circuit(3).add_gate(eng.GateType.H, 0).add_gate(eng.GateType.CNOT, 1, 0).serialize().encode()
+    assert L.moonlab_scheduler_set_completion_hook(hook, None) == 0
+    try:
+        with eng.Server() as srv:
+            rc, _ = eng.handle_frame(b"CIRCUIT %d\n" % len(text) + text, srv, tenant="acme-corp")
+            assert rc == eng.CONTROL_OK
+            rc, _ = eng.handle_frame(b"SHOTS 100 %d\n" % len(text) + text, srv, seed=5, tenant="globex")
+            assert rc == eng.CONTROL_OK
+            rc, _ = eng.handle_frame(b"SHOTS 10 %d\n" % len(text) + text, srv, seed=5)
+            assert rc == eng.CONTROL_OK
+            assert eng.handle_frame(b"HEALTH\n", srv, tenant="acme-corp")[0] == eng.CONTROL_OK
+            bad = b"NUM_QUBITS 2\nFOO 0\n"
+            assert eng.handle_frame(b"CIRCUIT %d\n" % len(bad) + bad, srv, tenant="acme-corp")[0] == \
+                eng.CONTROL_REJECTED
+        assert seen == [(False, "control_plane", 3, 0, b"acme-corp"),
+                        (False, "control_plane", 3, 100, b"globex"),
+                        (False, "control_plane", 3, 10, None)]
+        assert L.moonlab_scheduler_current_tenant_id() is None
+    finally:
+        L.moonlab_scheduler_set_completion_hook(eng.COMPLETION_HOOK_FN(), None)

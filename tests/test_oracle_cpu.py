@@ -46,6 +46,68 @@ def test_oracle_matches_reference_layers20(oracle):
     assert abs(oracle.norm_sq(n, a) - 1.0) < 1e-12
 
 
+def test_oracle_probabilities_match_reference(oracle, ref):
+    """svo_probabilities (measurement_probability_distribution,
+    measurement.c:140-161): the 64 largest probabilities and their sum against
+    the reference's own vector for the 20-qubit depth-20 circuit
+    (layers20_ref.npz), then elementwise against the compiled reference's
+    distribution on random circuits of the whole gate table.  The reference
+    squares cabs(a) (hypot); |a|^2 differs from it by a few ulp at most."""
+    f = np.load(os.path.join(GOLDEN, "layers20_ref.npz"))
+    n = 20
+    p = oracle.probabilities(n, oracle.run(n, f["gates"]))
+    top = np.sort(p)[-64:]
+    assert np.max(np.abs(top - f["prob_top"]) / f["prob_top"]) < 1e-12
+    assert abs(np.sum(p) - f["norm"]) < 1e-12
+    rng = np.random.default_rng(41)
+    for n in (1, 3, 7, 11):
+        g = gates_array(random_gate_list(n, 40, rng, allow3=False) if n > 1 else [(4, 0), (8, 0, -1, -1, 0.7)])
+        g = g[g["type"] <= 13]                   # the reference harness runs the 14 job-path codes
+        a, _, pr = ref.run(n, g, want_probs=True)
+        po = oracle.probabilities(n, a)
+        assert np.max(np.abs(po - pr)) <= 4 * np.finfo(float).eps * max(1.0, np.max(pr)), n
+        assert abs(np.sum(po) - 1.0) < 1e-13
+
+
+def test_oracle_fidelity_matches_reference(oracle, ref):
+    """svo_fidelity against the reference's quantum_state_fidelity
+    (state.c:289-306) on pairs of random circuit states, and the closed forms
+    |<psi|psi>|^2 = 1, |<0..0|+..+>|^2 = 2^-n, orthogonal states 0.  A missing
+    conjugate (sum a_i b_i) fails the random pairs."""
+    L = ref.L
+    L.quantum_state_fidelity.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.quantum_state_fidelity.restype = ctypes.c_double
+    rng = np.random.default_rng(43)
+    def circuit(n):
+        # a generic complex product state (RY, RZ, RX of random angles on
+        # every qubit), then random gates of the 14 job-path codes
+        rows = [(t, q, -1, -1, float(rng.uniform(0, 6.3))) for q in range(n) for t in (O.RY, O.RZ, O.RX)]
+        g = gates_array(rows + [tuple(x) for x in random_gate_list(n, 30, rng, allow3=False)])
+        return g[g["type"] <= 13]
+
+    for n in (2, 5, 9):
+        g1, g2 = circuit(n), circuit(n)
+        s1, s2 = ref.state(n, g1), ref.state(n, g2)
+        try:
+            want = L.quantum_state_fidelity(s1, s2)
+            a1 = np.ctypeslib.as_array(L.ref_state_amplitudes(s1), shape=(2 << n,)).view(np.complex128).copy()
+            a2 = np.ctypeslib.as_array(L.ref_state_amplitudes(s2), shape=(2 << n,)).view(np.complex128).copy()
+        finally:
+            ref.free(s1)
+            ref.free(s2)
+        got = oracle.fidelity(n, a1, a2)
+        assert abs(got - want) < 1e-14, n
+        assert abs(got - abs(np.vdot(a1, a2)) ** 2) < 1e-14
+        assert abs(got - abs(np.sum(a1 * a2)) ** 2) > 1e-6      # the conjugate matters here
+        assert abs(oracle.fidelity(n, a1, a1) - 1.0) < 1e-13
+    n = 6
+    zero = oracle.zero_state(n)
+    plus = oracle.run(n, [(O.H, q) for q in range(n)])
+    assert abs(oracle.fidelity(n, zero, plus) - 2.0 ** -n) < 1e-15
+    one = oracle.run(n, [(O.X, 0)])
+    assert oracle.fidelity(n, zero, one) == 0.0
+
+
 def test_oracle_job_shots_match_reference():
     """Per-worker seeds, slicing and the collapse outcome rule, bit-exact."""
     orc = O.Oracle()
