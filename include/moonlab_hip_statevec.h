@@ -216,6 +216,22 @@ MOONLAB_API moonlab_hip_status_t moonlab_hip_norm(const moonlab_hip_state_t *sta
  * pass over the state, accumulated in FP64 (either output may be NULL). */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_expect_z_all(const moonlab_hip_state_t *state,
                                                           double *out_z, double *out_norm);
+/* Addition with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the sums of moonlab_hip_expect_z_all on
+ * a state the caller owns, enqueued on `stream` (NULL = the default stream)
+ * with no allocation, no host copy and no synchronization.  `amplitudes` is
+ * the device address of 2^n_qubits complex values (precision
+ * MOONLAB_HIP_C128 or MOONLAB_HIP_C64) and `out` the device address of
+ * n_qubits + 1 doubles, both on the current device: out[0] = sum |a|^2,
+ * out[1 + q] = <Z_q> = 1 - 2 P(q = 1), bit-identical to
+ * moonlab_hip_expect_z_all.  The block partials live in a fixed 576 KiB
+ * device array per GPU (2048 blocks x 36 doubles), so n_qubits is limited to
+ * 34 (complex128) / 35 (complex64); larger values return
+ * MOONLAB_HIP_ERR_BAD_ARG.  Calls that overlap in time on one GPU share that
+ * array and race: serialize them (one stream, or events). */
+MOONLAB_API moonlab_hip_status_t moonlab_hip_expect_z_all_noalloc(const void *amplitudes, uint32_t n_qubits,
+                                                                  int precision, void *stream, double *out);
 /* Additions: measurement_expectation_x/_y and measurement_correlation_zz
  * (src/quantum/measurement.c:442-534), FP64 accumulation. */
 MOONLAB_API moonlab_hip_status_t moonlab_hip_expect_x(const moonlab_hip_state_t *state,

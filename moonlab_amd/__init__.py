@@ -320,6 +320,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_hip_prob_z": ([vp, u32, dp], ip),
         "moonlab_hip_norm": ([vp, dp], ip),
         "moonlab_hip_expect_z_all": ([vp, dp, dp], ip),
+        "moonlab_hip_expect_z_all_noalloc": ([vp, u32, ip, vp, vp], ip),
         "moonlab_hip_expect_x": ([vp, u32, dp], ip),
         "moonlab_hip_expect_y": ([vp, u32, dp], ip),
         "moonlab_hip_correlation_zz": ([vp, u32, u32, dp], ip),

@@ -33,6 +33,9 @@ struct GateTuning {
 };
 GateTuning &gate_tuning();
 
+// Record the text moonlab_hip_last_error() returns on this thread (returns st).
+moonlab_hip_status_t set_last_error(moonlab_hip_status_t st, const char *what, hipError_t e = hipSuccess);
+
 // Point a state's work at another stream (waits for the state's current
 // stream first); returns the stream it used before.
 hipStream_t state_bind_stream(moonlab_hip_state_t *s, hipStream_t st);

@@ -23,6 +23,8 @@
 
 #include <limits.h>
 
+#include <mutex>
+
 namespace mlx {
 namespace {
 
@@ -496,3 +498,101 @@ hipError_t launch_inner(const SvDev &a, const SvDev &b, double *d_partial, int b
 }
 
 }  // namespace mlx
+
+// ---------------------------------------------------------------------------
+// Norm and <Z_q> without allocating: the block partials of k_reduce_tiles /
+// k_reduce_small live in a fixed device array instead of a state's workspace.
+//
+// kNoAllocMaxQubits bounds the array: kMaxRedBlocks blocks x (n + 1) FP64
+// partials for n <= 35 (complex64's limit) is 2048 x 36 x 8 B = 576 KiB per
+// GPU, held from the first call on a device until the process ends.  Every
+// partial a call reads is written earlier in the same call, so nothing needs
+// clearing between calls.  All calls on one GPU share the array: calls that
+// overlap in time on one GPU (two streams, two threads) race on it.
+namespace {
+
+constexpr int kNoAllocMaxQubits = MOONLAB_HIP_MAX_QUBITS_C64;
+constexpr size_t kScratchDoubles = (size_t)mlx::kMaxRedBlocks * (kNoAllocMaxQubits + 1);
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices];   // per device: the array's device address
+
+// The array's address on the current device, looked up once per device with
+// that device current.  In host code `scratch` names a host-side stand-in,
+// never device memory: only this address may reach a kernel or a copy.
+hipError_t scratch_on_current_device(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+__global__ void k_z_from_sums(const double *__restrict__ partial, int blocks, int nvals, double *__restrict__ out)
+{
+    const int v = threadIdx.x;
+    if (v >= nvals) return;
+    double s = 0.0;
+    for (int b = 0; b < blocks; b++) s += partial[(size_t)b * nvals + v];
+    // 1 - 2 P(1): 2 s is exact, so this rounds once, as 1.0 - 2.0 * s does
+    out[v] = v == 0 ? s : __builtin_fma(-2.0, s, 1.0);
+}
+
+}  // namespace
+
+extern "C" moonlab_hip_status_t moonlab_hip_expect_z_all_noalloc(const void *amplitudes, uint32_t n_qubits,
+                                                                 int precision, void *stream, double *out)
+{
+    using namespace mlx;
+    if (!amplitudes || !out) return set_last_error(MOONLAB_HIP_ERR_BAD_ARG, "expect_z_all_noalloc: NULL pointer");
+    if (precision != MOONLAB_HIP_C128 && precision != MOONLAB_HIP_C64)
+        return set_last_error(MOONLAB_HIP_ERR_BAD_ARG, "expect_z_all_noalloc: precision is not C128 or C64");
+    const int cap = precision == MOONLAB_HIP_C64 ? MOONLAB_HIP_MAX_QUBITS_C64 : MOONLAB_HIP_MAX_QUBITS_C128;
+    if (n_qubits < 1 || (int)n_qubits > cap || (int)n_qubits > kNoAllocMaxQubits)
+        return set_last_error(MOONLAB_HIP_ERR_BAD_ARG,
+                              "expect_z_all_noalloc: n_qubits out of range (1..34 complex128, 1..35 complex64; "
+                              "the fixed partial-sum array holds 2048 x 36 doubles)");
+    double *partial = nullptr;
+    hipError_t e = scratch_on_current_device(&partial);
+    if (e != hipSuccess) return set_last_error(MOONLAB_HIP_ERR_DRIVER, "expect_z_all_noalloc: scratch lookup", e);
+    const int n = (int)n_qubits;
+    int tile_bits = 0, blocks = 0;
+    uint64_t ntiles = 0;
+    reduce_geometry(n, &tile_bits, &ntiles, &blocks);
+    const hipStream_t st = (hipStream_t)stream;
+    // the same kernels, grid and partial layout as moonlab_hip_expect_z_all,
+    // so the sums are bit-identical
+    if (n < kTileBits) {
+        if (precision == MOONLAB_HIP_C64)
+            k_reduce_small<float><<<1, kRedThreads, 0, st>>>((const float2 *)amplitudes, n, partial, nullptr);
+        else
+            k_reduce_small<double><<<1, kRedThreads, 0, st>>>((const double2 *)amplitudes, n, partial, nullptr);
+    } else {
+        if (precision == MOONLAB_HIP_C64)
+            k_reduce_tiles<float><<<blocks, kRedThreads, 0, st>>>((const float2 *)amplitudes, n, ntiles, partial,
+                                                                  nullptr);
+        else
+            k_reduce_tiles<double><<<blocks, kRedThreads, 0, st>>>((const double2 *)amplitudes, n, ntiles,
+                                                                   partial, nullptr);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_z_from_sums<<<1, 64, 0, st>>>(partial, blocks, n + 1, out);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return set_last_error(MOONLAB_HIP_ERR_DRIVER, "expect_z_all_noalloc: launch", e);
+    return MOONLAB_HIP_OK;
+}

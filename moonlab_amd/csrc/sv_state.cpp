@@ -138,6 +138,11 @@ moonlab_hip_status_t check_gate(const moonlab_hip_state_t *s, const moonlab_hip_
 
 namespace mlx {
 
+moonlab_hip_status_t set_last_error(moonlab_hip_status_t st, const char *what, hipError_t e)
+{
+    return fail(st, what, e);
+}
+
 hipStream_t state_bind_stream(moonlab_hip_state_t *s, hipStream_t st)
 {
     hipStream_t old = s->stream;
