@@ -101,6 +101,17 @@ class Oracle:
         L.svo_compose_unitary.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp,
                                           ctypes.c_int, dp]
         L.svo_compose_unitary.restype = ctypes.c_int
+        cpp = ctypes.POINTER(ctypes.c_char_p)
+        L.svo_vqe_state.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_uint64, dp]
+        L.svo_vqe_energy.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                                     cpp, dp, ctypes.c_double]
+        L.svo_vqe_energy.restype = ctypes.c_double
+        L.svo_vqe_gradient.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_uint64,
+                                       ctypes.c_int, cpp, dp, dp]
+        L.svo_vqe_qgt.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_uint64, dp, dp]
+        L.svo_vqe_qgt.restype = ctypes.c_int
+        L.svo_solve.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.svo_solve.restype = ctypes.c_int
         self.L = L
 
     # -- state evolution -------------------------------------------------
@@ -212,6 +223,50 @@ class Oracle:
             a[k] += np.pi / 2
             g[k] = 0.5 * (ep - em)
         return g.reshape(layers, n, 2)
+
+    # -- variational: Pauli sums, gradients, geometric tensor ---------------
+    @staticmethod
+    def _terms(terms):
+        strs = (ctypes.c_char_p * max(len(terms), 1))(*[t[1].encode() for t in terms])
+        coef = np.ascontiguousarray([t[0] for t in terms] or [0.0], dtype=np.float64)
+        return strs, coef
+
+    def vqe_state(self, n, layers, angles, entangler=1, hf=0):
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
+        out = np.empty(1 << n, dtype=np.complex128)
+        self.L.svo_vqe_state(n, layers, _ptr(a), entangler, hf, _ptr(out.view(np.float64)))
+        return out
+
+    def vqe_energy(self, n, layers, angles, terms, entangler=1, hf=0, constant=0.0):
+        """terms: (coefficient, pauli string) pairs; string[q] acts on qubit q."""
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
+        strs, coef = self._terms(terms)
+        return self.L.svo_vqe_energy(n, layers, _ptr(a), entangler, hf, len(terms), strs, _ptr(coef), constant)
+
+    def vqe_gradient(self, n, layers, angles, terms, entangler=1, hf=0):
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
+        strs, coef = self._terms(terms)
+        g = np.zeros(a.size)
+        self.L.svo_vqe_gradient(n, layers, _ptr(a), entangler, hf, len(terms), strs, _ptr(coef), _ptr(g))
+        return g
+
+    def vqe_qgt(self, n, layers, angles, entangler=1, hf=0):
+        """(metric, Berry curvature), each P x P with P = 2 n layers."""
+        a = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
+        P = 2 * n * layers
+        g = np.zeros((P, P))
+        f = np.zeros((P, P))
+        if self.L.svo_vqe_qgt(n, layers, _ptr(a), entangler, hf, _ptr(g), _ptr(f)):
+            raise MemoryError("svo_vqe_qgt failed")
+        return g, f
+
+    def solve(self, A, b):
+        A = np.array(A, dtype=np.float64, order="C")
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(b.size)
+        if self.L.svo_solve(b.size, _ptr(A), _ptr(b), _ptr(x)):
+            raise np.linalg.LinAlgError("singular")
+        return x
 
     def compose_unitary(self, qubits, gates):
         g = gates if isinstance(gates, np.ndarray) else gates_array(gates)
@@ -327,3 +382,69 @@ class RefLib:
 
     def free(self, s):
         self.L.ref_state_delete(s)
+
+    # -- the reference's own VQE solver (vqe.c, vqe_qng.c) over its
+    #    hardware-efficient ansatz (CNOT ladder) -------------------------------
+    def vqe(self, n, layers, terms, hf=0, constant=0.0):
+        L = self.L
+        if not getattr(L, "_vqe_declared", False):
+            dp = ctypes.POINTER(ctypes.c_double)
+            vp = ctypes.c_void_p
+            L.ref_vqe_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                      dp, ctypes.c_uint64, ctypes.c_double]
+            L.ref_vqe_new.restype = vp
+            L.ref_vqe_free.argtypes = [vp]
+            L.ref_vqe_energy.argtypes = [vp, dp]
+            L.ref_vqe_energy.restype = ctypes.c_double
+            for f in ("ref_vqe_gradient", "ref_vqe_qgt", "ref_vqe_berry"):
+                getattr(L, f).argtypes = [vp, dp, dp]
+                getattr(L, f).restype = ctypes.c_int
+            L.ref_vqe_natural_gradient.argtypes = [vp, dp, dp, ctypes.c_double, dp]
+            L.ref_vqe_natural_gradient.restype = ctypes.c_int
+            L._vqe_declared = True
+        return RefVqe(L, n, layers, terms, hf, constant)
+
+
+class RefVqe:
+    """One reference vqe_solver_t (test infrastructure)."""
+
+    def __init__(self, L, n, layers, terms, hf, constant):
+        self.L, self.P = L, 2 * n * layers
+        strs = (ctypes.c_char_p * len(terms))(*[t[1].encode() for t in terms])
+        coef = np.ascontiguousarray([t[0] for t in terms], dtype=np.float64)
+        self.h = L.ref_vqe_new(n, layers, len(terms), strs, _ptr(coef), hf, constant)
+        if not self.h:
+            raise ValueError("ref_vqe_new failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_vqe_free(self.h)
+            self.h = None
+
+    def _p(self, params):
+        return np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+
+    def energy(self, params):
+        return self.L.ref_vqe_energy(self.h, _ptr(self._p(params)))
+
+    def _mat(self, f, params, shape):
+        out = np.zeros(shape)
+        if f(self.h, _ptr(self._p(params)), _ptr(out)):
+            raise ValueError("reference call failed")
+        return out
+
+    def gradient(self, params):
+        return self._mat(self.L.ref_vqe_gradient, params, (self.P,))
+
+    def qgt(self, params):
+        return self._mat(self.L.ref_vqe_qgt, params, (self.P, self.P))
+
+    def berry(self, params):
+        return self._mat(self.L.ref_vqe_berry, params, (self.P, self.P))
+
+    def natural_gradient(self, params, grad, reg):
+        out = np.zeros(self.P)
+        g = np.ascontiguousarray(grad, dtype=np.float64)
+        if self.L.ref_vqe_natural_gradient(self.h, _ptr(self._p(params)), _ptr(g), reg, _ptr(out)):
+            raise np.linalg.LinAlgError("singular")
+        return out

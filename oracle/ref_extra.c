@@ -100,3 +100,71 @@ int ref_run(int n, const ref_gate_t *g, int ng, double *amps_out, double *z_out,
     ref_state_delete(s);
     return rc;
 }
+
+/* ---- VQE glue: the reference's solver over its hardware-efficient ansatz
+ * (vqe.c:764-781: RY, RZ per qubit, then CNOT(q, q+1)) and a Pauli-sum
+ * Hamiltonian given as strings (pauli_string[q] acts on qubit q), with the
+ * Hartree-Fock bits and the constant (nuclear repulsion) of
+ * pauli_hamiltonian_t.  Drives vqe_compute_energy / _gradient / _qgt /
+ * _berry_curvature / vqe_natural_gradient_direction (vqe.c, vqe_qng.c). */
+#include "algorithms/vqe.h"
+
+typedef struct {
+    pauli_hamiltonian_t *h;
+    vqe_ansatz_t *a;
+    vqe_optimizer_t *o;
+    quantum_entropy_ctx_t ent;
+    vqe_solver_t *s;
+} ref_vqe_t;
+
+static int ref_vqe_zero_bytes(void *user, uint8_t *buf, size_t n)
+{
+    (void)user;
+    memset(buf, 0, n);
+    return 0;
+}
+
+ref_vqe_t *ref_vqe_new(int n, int layers, int nterms, const char *const *strings, const double *coeffs,
+                       uint64_t hf_reference, double constant)
+{
+    ref_vqe_t *v = (ref_vqe_t *)calloc(1, sizeof(*v));
+    if (!v) return NULL;
+    v->h = pauli_hamiltonian_create((size_t)n, (size_t)nterms);
+    v->a = vqe_create_hardware_efficient_ansatz((size_t)n, (size_t)layers);
+    v->o = vqe_optimizer_create(VQE_OPTIMIZER_GRADIENT_DESCENT);
+    if (!v->h || !v->a || !v->o) goto fail;
+    for (int k = 0; k < nterms; k++)
+        if (pauli_hamiltonian_add_term(v->h, coeffs[k], strings[k], (size_t)k) != 0) goto fail;
+    v->h->hf_reference = hf_reference;
+    v->h->nuclear_repulsion = constant;
+    quantum_entropy_init(&v->ent, ref_vqe_zero_bytes, NULL);
+    v->s = vqe_solver_create(v->h, v->a, v->o, &v->ent);
+    if (!v->s) goto fail;
+    return v;
+fail:
+    if (v->h) pauli_hamiltonian_free(v->h);
+    if (v->a) vqe_ansatz_free(v->a);
+    if (v->o) vqe_optimizer_free(v->o);
+    free(v);
+    return NULL;
+}
+
+void ref_vqe_free(ref_vqe_t *v)
+{
+    if (!v) return;
+    vqe_solver_free(v->s);
+    pauli_hamiltonian_free(v->h);
+    vqe_ansatz_free(v->a);
+    vqe_optimizer_free(v->o);
+    free(v);
+}
+
+int ref_vqe_num_parameters(const ref_vqe_t *v) { return (int)v->a->num_parameters; }
+double ref_vqe_energy(ref_vqe_t *v, const double *p) { return vqe_compute_energy(v->s, p); }
+int ref_vqe_gradient(ref_vqe_t *v, const double *p, double *g) { return vqe_compute_gradient(v->s, p, g); }
+int ref_vqe_qgt(ref_vqe_t *v, const double *p, double *out) { return vqe_compute_qgt(v->s, p, out); }
+int ref_vqe_berry(ref_vqe_t *v, const double *p, double *out) { return vqe_compute_berry_curvature(v->s, p, out); }
+int ref_vqe_natural_gradient(ref_vqe_t *v, const double *p, const double *g, double reg, double *out)
+{
+    return vqe_natural_gradient_direction(v->s, p, g, reg, out);
+}

@@ -753,6 +753,166 @@ double svo_hea_tfim_energy(int n, int layers, const double *ang, double h, doubl
 }
 
 /* ---------------------------------------------------------------------------
+ * Variational energies, gradients and the quantum geometric tensor.
+ * ------------------------------------------------------------------------- */
+
+static void vqe_ansatz(int n, int layers, const double *ang, int entangler, uint64_t hf, double *amps,
+                       long insert_at, int insert_gen)
+{
+    /* vqe_compute_energy's state preparation (vqe.c:1302-1318) over the
+     * hardware-efficient ansatz (vqe.c:764-781); with insert_at >= 0 the
+     * generator of parameter insert_at (Y after RY, Z after RZ) is applied
+     * right after its rotation, qng_hea_state_derivatives (vqe_qng.c:66-103). */
+    svo_init_zero(n, amps);
+    for (int q = 0; q < n; q++)
+        if ((hf >> q) & 1ULL) g_x(n, amps, q);
+    long p = 0;
+    for (int l = 0; l < layers; l++) {
+        for (int q = 0; q < n; q++) {
+            g_ry(n, amps, q, ang[p]);
+            if (p == insert_at && insert_gen) g_y(n, amps, q);
+            p++;
+            g_rz(n, amps, q, ang[p]);
+            if (p == insert_at && insert_gen) g_z(n, amps, q);
+            p++;
+        }
+        for (int q = 0; q + 1 < n; q++) {
+            if (entangler == 1) g_cnot(n, amps, q, q + 1);
+            else g_cz(n, amps, q, q + 1);
+        }
+    }
+}
+
+void svo_vqe_state(int n, int layers, const double *ang, int entangler, uint64_t hf, double *amps)
+{
+    vqe_ansatz(n, layers, ang, entangler, hf, amps, -1, 0);
+}
+
+double svo_vqe_energy(int n, int layers, const double *ang, int entangler, uint64_t hf, int nterms,
+                      const char *const *strings, const double *coeffs, double constant)
+{
+    const size_t dim = (size_t)1 << n;
+    double *a = (double *)malloc(dim * 2 * sizeof(double));
+    if (!a) return NAN;
+    vqe_ansatz(n, layers, ang, entangler, hf, a, -1, 0);
+    double e = 0.0;   /* term order, then the constant (vqe.c:1326-1345) */
+    for (int k = 0; k < nterms; k++) e += coeffs[k] * svo_expect_pauli(n, a, strings[k]);
+    e += constant;
+    free(a);
+    return e;
+}
+
+void svo_vqe_gradient(int n, int layers, const double *ang, int entangler, uint64_t hf, int nterms,
+                      const char *const *strings, const double *coeffs, double *grad)
+{
+    const int P = 2 * n * layers;
+    double *t = (double *)malloc((size_t)(P > 0 ? P : 1) * sizeof(double));
+    if (!t) return;
+    for (int k = 0; k < P; k++) {
+        memcpy(t, ang, (size_t)P * sizeof(double));
+        t[k] = ang[k] + M_PI / 2.0;
+        const double ep = svo_vqe_energy(n, layers, t, entangler, hf, nterms, strings, coeffs, 0.0);
+        t[k] = ang[k] - M_PI / 2.0;
+        const double em = svo_vqe_energy(n, layers, t, entangler, hf, nterms, strings, coeffs, 0.0);
+        grad[k] = 0.5 * (ep - em);
+    }
+    free(t);
+}
+
+int svo_vqe_qgt(int n, int layers, const double *ang, int entangler, uint64_t hf, double *qgt, double *berry)
+{
+    const int P = 2 * n * layers;
+    const size_t dim = (size_t)1 << n;
+    double *psi = (double *)malloc(dim * 2 * sizeof(double));
+    double *d = (double *)malloc((size_t)(P > 0 ? P : 1) * dim * 2 * sizeof(double));
+    double *ov = (double *)malloc((size_t)(P > 0 ? P : 1) * 2 * sizeof(double));
+    if (!psi || !d || !ov) {
+        free(psi); free(d); free(ov);
+        return -1;
+    }
+    vqe_ansatz(n, layers, ang, entangler, hf, psi, -1, 0);
+    for (int i = 0; i < P; i++) {
+        double *di = d + (size_t)i * dim * 2;
+        vqe_ansatz(n, layers, ang, entangler, hf, di, i, 1);
+        for (size_t x = 0; x < dim; x++) {   /* (-i/2) (re + i im) = (im / 2) - i (re / 2) */
+            const double re = di[2 * x], im = di[2 * x + 1];
+            di[2 * x] = 0.5 * im;
+            di[2 * x + 1] = -0.5 * re;
+        }
+        double sr = 0.0, si = 0.0;   /* <psi|d_i psi> */
+        for (size_t x = 0; x < dim; x++) {
+            sr += psi[2 * x] * di[2 * x] + psi[2 * x + 1] * di[2 * x + 1];
+            si += psi[2 * x] * di[2 * x + 1] - psi[2 * x + 1] * di[2 * x];
+        }
+        ov[2 * i] = sr;
+        ov[2 * i + 1] = si;
+    }
+    for (int i = 0; i < P; i++) {
+        const double *di = d + (size_t)i * dim * 2;
+        for (int j = i; j < P; j++) {
+            const double *dj = d + (size_t)j * dim * 2;
+            double ar = 0.0, ai = 0.0;   /* <d_i psi|d_j psi> */
+            for (size_t x = 0; x < dim; x++) {
+                ar += di[2 * x] * dj[2 * x] + di[2 * x + 1] * dj[2 * x + 1];
+                ai += di[2 * x] * dj[2 * x + 1] - di[2 * x + 1] * dj[2 * x];
+            }
+            /* conj(ov_i) ov_j */
+            const double br = ov[2 * i] * ov[2 * j] + ov[2 * i + 1] * ov[2 * j + 1];
+            const double bi = ov[2 * i] * ov[2 * j + 1] - ov[2 * i + 1] * ov[2 * j];
+            if (qgt) {
+                qgt[(size_t)i * P + j] = ar - br;
+                qgt[(size_t)j * P + i] = ar - br;
+            }
+            if (berry) {
+                const double f = i == j ? 0.0 : -2.0 * (ai - bi);
+                berry[(size_t)i * P + j] = f;
+                berry[(size_t)j * P + i] = -f;
+            }
+        }
+    }
+    free(psi); free(d); free(ov);
+    return 0;
+}
+
+int svo_solve(int m, double *A, const double *b, double *x)
+{
+    double *r = (double *)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    if (!r) return -1;
+    memcpy(r, b, (size_t)m * sizeof(double));
+    for (int c = 0; c < m; c++) {
+        int piv = c;
+        for (int i = c + 1; i < m; i++)
+            if (fabs(A[(size_t)i * m + c]) > fabs(A[(size_t)piv * m + c])) piv = i;
+        if (!(fabs(A[(size_t)piv * m + c]) > 0.0)) {
+            free(r);
+            return -1;
+        }
+        if (piv != c) {
+            for (int k = 0; k < m; k++) {
+                const double t = A[(size_t)c * m + k];
+                A[(size_t)c * m + k] = A[(size_t)piv * m + k];
+                A[(size_t)piv * m + k] = t;
+            }
+            const double t = r[c];
+            r[c] = r[piv];
+            r[piv] = t;
+        }
+        for (int i = c + 1; i < m; i++) {
+            const double f = A[(size_t)i * m + c] / A[(size_t)c * m + c];
+            for (int k = c; k < m; k++) A[(size_t)i * m + k] -= f * A[(size_t)c * m + k];
+            r[i] -= f * r[c];
+        }
+    }
+    for (int i = m - 1; i >= 0; i--) {
+        double s = r[i];
+        for (int k = i + 1; k < m; k++) s -= A[(size_t)i * m + k] * x[k];
+        x[i] = s / A[(size_t)i * m + i];
+    }
+    free(r);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
  * Fused-block composition (brute force, FP64).
  * ------------------------------------------------------------------------- */
 

@@ -106,6 +106,31 @@ int svo_job_shots(int n, const double *amps, int total_shots, int num_workers,
 double svo_hea_tfim_energy(int n, int layers, const double *angles, double h,
                            double *amps_scratch);
 
+/* Variational energies over a Pauli-sum Hamiltonian (vqe.c:1287-1350): the
+ * Hartree-Fock bits set by X (vqe.c:1302-1305), then per layer RY, RZ on
+ * every qubit and the entangling ladder over (q, q+1) -- entangler 1: CNOT
+ * with control q (the reference ansatz, vqe.c:764-781), 0: CZ -- then
+ * E = sum_k c_k <P_k> (svo_expect_pauli, term order) + constant.
+ * angles: layers * n * 2 doubles, [layer][qubit][ry, rz].  strings[k][q] in
+ * "IXYZ" acts on qubit q. */
+void   svo_vqe_state(int n, int layers, const double *angles, int entangler, uint64_t hf, double *amps);
+double svo_vqe_energy(int n, int layers, const double *angles, int entangler, uint64_t hf, int nterms,
+                      const char *const *strings, const double *coeffs, double constant);
+/* Parameter-shift gradient, dE/dtheta_k = [E(theta + pi/2 e_k) - E(theta -
+ * pi/2 e_k)] / 2 (vqe.c:2094-2130). */
+void   svo_vqe_gradient(int n, int layers, const double *angles, int entangler, uint64_t hf, int nterms,
+                        const char *const *strings, const double *coeffs, double *grad);
+/* The quantum geometric tensor of the ansatz state (vqe_qng.c:66-170): exact
+ * derivatives d_i psi = (-i/2) U_after P_i U_upto |HF> (P_i = Y after RY, Z
+ * after RZ), ov_i = <psi|d_i psi>, Q_ij = <d_i psi|d_j psi> - <d_i psi|psi>
+ * <psi|d_j psi>; qgt = Re Q (vqe_qng.c:172-205), berry = -2 Im Q (:207-245),
+ * each P x P row-major (P = 2 n layers), either may be NULL. */
+int    svo_vqe_qgt(int n, int layers, const double *angles, int entangler, uint64_t hf, double *qgt,
+                   double *berry);
+/* Solve (A) x = b for an m x m row-major A (overwritten): Gaussian
+ * elimination with partial pivoting; -1 when a pivot vanishes. */
+int    svo_solve(int m, double *A, const double *b, double *x);
+
 /* FP64 composition of a list of gates acting only on `qubits` (k <= 5,
  * ascending) into a 2^k x 2^k row-major complex matrix in the local basis
  * (local bit j = qubits[j]).  Brute force: applies the gates column by

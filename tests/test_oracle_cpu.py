@@ -423,3 +423,81 @@ def test_oracle_parameter_shift_gradient_pinned(oracle):
             em = oracle.hea_tfim_energy(n, L, a, h)
             fd[idx] = (ep - em) / (2 * eps)
         assert np.max(np.abs(g - fd)) < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# variational energies, gradients, geometric tensor (vqe.c, vqe_qng.c)
+# ---------------------------------------------------------------------------
+
+def _random_hamiltonian(n, rng, count):
+    terms = []
+    for _ in range(count):
+        s = "".join(rng.choice(list("IXYZ"), size=n))
+        terms.append((float(rng.uniform(-1, 1)), s))
+    terms.append((0.5, "I" * n))
+    return terms
+
+
+def test_oracle_vqe_matches_reference_solver(oracle, ref):
+    """svo_vqe_energy / _gradient / _qgt and the natural-gradient solve
+    against the compiled reference's vqe_compute_energy, vqe_compute_gradient,
+    vqe_compute_qgt, vqe_compute_berry_curvature and
+    vqe_natural_gradient_direction on its hardware-efficient ansatz (CNOT
+    ladder), with Hartree-Fock bits and a constant."""
+    rng = np.random.default_rng(61)
+    for n, L in ((1, 1), (2, 2), (3, 1), (4, 2), (5, 3)):
+        terms = _random_hamiltonian(n, rng, 6)
+        hf = int(rng.integers(0, 1 << n))
+        ang = rng.uniform(0, 2 * np.pi, 2 * n * L)
+        v = ref.vqe(n, L, terms, hf=hf, constant=0.25)
+        assert abs(oracle.vqe_energy(n, L, ang, terms, 1, hf, 0.25) - v.energy(ang)) < 1e-13
+        g = oracle.vqe_gradient(n, L, ang, terms, 1, hf)
+        assert np.max(np.abs(g - v.gradient(ang))) < 1e-12
+        qg, qf = oracle.vqe_qgt(n, L, ang, 1, hf)
+        assert np.max(np.abs(qg - v.qgt(ang))) < 1e-14
+        assert np.max(np.abs(qf - v.berry(ang))) < 1e-14
+        A = qg + 1e-3 * np.eye(qg.shape[0])
+        ng = oracle.solve(A, g)
+        want = v.natural_gradient(ang, g, 1e-3)
+        assert np.max(np.abs(ng - want)) <= 1e-9 * max(1.0, np.max(np.abs(want))), (n, L)
+        assert np.max(np.abs(ng - np.linalg.solve(A, g))) <= 1e-9 * max(1.0, np.max(np.abs(want)))
+
+
+def test_oracle_qgt_closed_form_and_finite_differences(oracle):
+    """One qubit, |psi> = RZ(phi) RY(theta)|0>: metric diag(1/4, sin^2(theta)/4)
+    and Berry curvature F_theta,phi = -sin(theta)/2 (the -2 Im convention).
+    Then the metric and curvature of a 3-qubit CZ-ladder ansatz against
+    central differences of the prepared state (<d_i psi|d_j psi> -
+    <d_i psi|psi><psi|d_j psi> from differenced states)."""
+    for theta, phi in ((0.3, 1.1), (2.0, -0.7), (np.pi / 2, 0.0)):
+        g, f = oracle.vqe_qgt(1, 1, [theta, phi], 0, 0)
+        assert np.allclose(g, [[0.25, 0.0], [0.0, np.sin(theta) ** 2 / 4]], atol=1e-15)
+        assert np.allclose(f, [[0.0, -np.sin(theta) / 2], [np.sin(theta) / 2, 0.0]], atol=1e-15)
+    rng = np.random.default_rng(67)
+    n, L = 3, 2
+    ang = rng.uniform(0, 2 * np.pi, 2 * n * L)
+    psi = oracle.vqe_state(n, L, ang, 0, 0b010)
+    d = []
+    h = 1e-5
+    for i in range(ang.size):
+        a1, a2 = ang.copy(), ang.copy()
+        a1[i] += h
+        a2[i] -= h
+        d.append((oracle.vqe_state(n, L, a1, 0, 0b010) - oracle.vqe_state(n, L, a2, 0, 0b010)) / (2 * h))
+    d = np.array(d)
+    ov = d.conj() @ psi
+    Q = d.conj() @ d.T - np.outer(ov, ov.conj())
+    g, f = oracle.vqe_qgt(n, L, ang, 0, 0b010)
+    assert np.max(np.abs(g - Q.real)) < 1e-8
+    assert np.max(np.abs(f - (-2 * Q.imag))) < 1e-8
+    assert np.max(np.abs(f)) > 1e-2                 # the RZ layers make the curvature nonzero
+
+
+def test_oracle_solve_vs_numpy(oracle):
+    rng = np.random.default_rng(71)
+    for m in (1, 2, 7, 30):
+        A = rng.normal(size=(m, m)) + m * np.eye(m)
+        b = rng.normal(size=m)
+        assert np.max(np.abs(oracle.solve(A, b) - np.linalg.solve(A, b))) < 1e-12
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle.solve(np.zeros((3, 3)), np.ones(3))
