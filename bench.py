@@ -34,6 +34,7 @@ if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
 
 import numpy as np  # noqa: E402
 
+T_START = time.perf_counter()
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
@@ -61,11 +62,41 @@ def max_amp_error(m, n: int, gates: np.ndarray, ref_amps) -> float:
     return worst
 
 
-def cpu_baseline(n: int, gates: np.ndarray, budget_s: float, m=None) -> dict | None:
-    """The reference's C (oracle/_ref) on the host: a prefix of the same sweep,
-    gate by gate, until `budget_s` of CPU work; state init excluded.  With the
-    engine module `m`, the GPU's state after the same prefix is compared with
-    the reference's amplitude by amplitude."""
+def host_info() -> dict:
+    """CPU model, logical CPUs, the CPUs this process may use and host RAM."""
+    model, mem = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    mem = int(line.split()[1]) * 1024
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+            "host_ram_GiB": round(mem / 2 ** 30, 1) if mem else None}
+
+
+def family_counts(gates: np.ndarray) -> dict:
+    names = {4: "H", 7: "RX", 8: "RY", 9: "RZ", 10: "CNOT"}
+    out = {}
+    for t in gates["type"]:
+        k = names.get(int(t), str(int(t)))
+        out[k] = out.get(k, 0) + 1
+    return out
+
+
+def cpu_baseline(n: int, gates: np.ndarray, prefix: int, m=None) -> dict | None:
+    """The reference's C (oracle/_ref) on the host: the first `prefix` gates
+    of the same sweep (the H layer and the first full layer: rotations and
+    CNOTs), gate by gate; state init excluded.  With the engine module `m`,
+    the GPU's state after the same prefix is compared with the reference's
+    amplitude by amplitude over the whole vector."""
     try:
         from oracle.oracle import RefLib, gates_array
         from moonlab_amd.workloads import algorithmic_bytes
@@ -77,14 +108,17 @@ def cpu_baseline(n: int, gates: np.ndarray, budget_s: float, m=None) -> dict | N
     if not s:
         return None
     try:
-        ga = gates_array([tuple(g) for g in gates])
+        ga = gates_array([tuple(g) for g in gates[:prefix]])
         done = 0
-        t0 = time.perf_counter()
-        while done < len(ga) and time.perf_counter() - t0 < budget_s:
+        t0 = last = time.perf_counter()
+        while done < len(ga):
             rc = ref.L.ref_apply_gate(s, ga[done:done + 1].ctypes.data)
             if rc:
                 raise RuntimeError(f"reference gate {done} failed ({rc})")
             done += 1
+            if time.perf_counter() - last > 10.0:
+                last = time.perf_counter()
+                print(f"[bench] sweep cpu baseline: {done}/{len(ga)} gates", file=sys.stderr, flush=True)
         dt = time.perf_counter() - t0
         err = None
         if m is not None and done > 0:
@@ -97,11 +131,14 @@ def cpu_baseline(n: int, gates: np.ndarray, budget_s: float, m=None) -> dict | N
     b = algorithmic_bytes(gates[:done], n)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(b / dt / 1e12, 6), "unit": "TB/s", "cores": cores, "kind": "reference",
-            "sample": f"first {done} of {len(gates)} gates of the same {n}-qubit sweep "
-                      f"({b / 1e9:.1f} GB algorithmic, {dt:.1f} s), tsotchke/moonlab C "
-                      f"(gates.c stride-block loops, OpenMP) built by oracle/Makefile",
+            "sample": f"first {done} of {len(gates)} gates of the same {n}-qubit sweep (the H layer and the "
+                      f"first full layer: {b / 1e9:.1f} GB algorithmic, {dt:.1f} s), tsotchke/moonlab C "
+                      f"(gates.c stride-block loops, OpenMP on {cores} threads) built by oracle/Makefile",
             "gates_per_s": round(done / dt, 3),
-            "max_abs_amp_error_gpu_vs_reference": err}
+            "host": host_info(),
+            "gates_by_family": family_counts(gates[:done]),
+            "max_abs_amp_error_gpu_vs_reference": err,
+            "amp_error_over": f"all 2^{n} amplitudes after the {done}-gate prefix"}
 
 
 def batch_cpu_baseline(inst, budget_s: float) -> dict | None:
@@ -135,7 +172,51 @@ def batch_cpu_baseline(inst, budget_s: float) -> dict | None:
             "energies": energies}
 
 
-def run_batch_workload(m, count: int, steps: int, warmup: int, cpu_seconds: float, with_cpu: bool) -> dict:
+FP64_PEAK_TFLOPS = 78.6
+
+
+def hea_flops(inst) -> float:
+    """Algorithmic FP64 flops of the HEA/TFIM batch: each RY and RZ a dense
+    complex 2x2 on 2^(n-1) pairs (4 complex multiply-adds, 28 flops per pair),
+    CZ sign flips counted as free, and the observables: |a|^2 (3 flops) plus
+    n - 1 ZZ accumulations per amplitude, and 2 Re(a_i* a_j) (4 flops) per pair
+    for each <X_q>."""
+    f = 0.0
+    for n, L, _, _ in inst:
+        D = 1 << n
+        f += 2 * L * n * (D // 2) * 28 + D * 3 + D * (n - 1) + n * (D // 2) * 4
+    return f
+
+
+def batch_plain_gpu(m, inst, sample: int, batched_e) -> dict:
+    """Plain per-circuit dispatch of the same instances on the same GPU: one
+    device state per qubit count, reset, one kernel per gate, then the TFIM
+    terms through the single-state reductions (one per term), circuit after
+    circuit, on the first `sample` instances in batch order."""
+    from moonlab_amd import workloads
+    states = {}
+    got = []
+    for n, L, ang, h in inst[:8]:                     # warm the code objects and states
+        st = states.setdefault(n, m.StateVector(n, m.C128))
+        st.reset().apply(workloads.hea_gate_list(n, L, ang))
+        st.synchronize()
+    t0 = time.perf_counter()
+    for n, L, ang, h in inst[:sample]:
+        st = states.setdefault(n, m.StateVector(n, m.C128))
+        st.reset().apply(workloads.hea_gate_list(n, L, ang))
+        e = -sum(st.correlation_zz(q, q + 1) for q in range(n - 1)) - h * sum(st.expect_x(q) for q in range(n))
+        got.append(e)
+    dt = time.perf_counter() - t0
+    for st in states.values():
+        st.free()
+    return {"circuits_per_s": round(sample / dt, 1), "sample": f"first {sample} of {len(inst)} instances in batch "
+            "order, one circuit at a time: state reset, one kernel per gate, one reduction per TFIM term",
+            "seconds": round(dt, 3),
+            "max_abs_energy_diff_vs_batched": float(np.max(np.abs(np.array(got) - batched_e[:sample])))}
+
+
+def run_batch_workload(m, count: int, steps: int, warmup: int, cpu_seconds: float, with_cpu: bool,
+                       plain_sample: int = 512) -> dict:
     from moonlab_amd import workloads
     inst = workloads.hea_batch(count, SEED)
     b = m.Batch()
@@ -164,6 +245,15 @@ def run_batch_workload(m, count: int, steps: int, warmup: int, cpu_seconds: floa
            "launches_per_batch": 1,
            "autotune": {"tile_bits": tuned["tile_bits"], "workgroups": tuned["workgroups"]},
            "hbm_passes_per_batch": tuned["total_passes"]}
+    flops = hea_flops(inst)
+    dev = float(np.mean(devs))
+    out["roofline"] = {"bound": "fp64 issue (circuits resident in LDS; see profiles/k_batch_hea4096.md)",
+                       "kernel": "k_batch", "achieved": round(flops / dev / 1e12, 2), "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(flops / dev / 1e12 / FP64_PEAK_TFLOPS, 4),
+                       "algorithmic_flops_per_batch": flops}
+    if plain_sample > 0:
+        out["plain_per_circuit_gpu"] = batch_plain_gpu(m, inst, min(plain_sample, count), e)
+        out["batched_vs_plain_gpu"] = round(out["circuits_per_s"] / out["plain_per_circuit_gpu"]["circuits_per_s"], 1)
     if with_cpu:
         cpu = batch_cpu_baseline(inst, cpu_seconds)
         if cpu:
@@ -482,8 +572,13 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-count", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--serve-sweep", default="50,200,400",
+    ap.add_argument("--serve-sweep", default="200,400",
                     help="other arrival rates (per s) at which the served leg runs again")
+    ap.add_argument("--no-fusion", action="store_true", help="leave out the fusion leg of the default line")
+    ap.add_argument("--batch-plain-sample", type=int, default=512,
+                    help="instances of the batch run one at a time for the plain per-circuit GPU leg")
+    ap.add_argument("--time-budget", type=float, default=540.0,
+                    help="seconds the whole default run may take: legs past it are skipped and named")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--workload", choices=("sweep", "fusion", "blocked"), default="sweep",
                     help="sweep: the 30-qubit gate sweep (with the batch and serving legs); "
@@ -617,24 +712,49 @@ def main() -> int:
         traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
 
     value = bytes_step * K / wall / 1e12
-    cpu = None if args.no_cpu else cpu_baseline(n, gates, args.cpu_seconds, m)
+    skipped = {}
+
+    def fits(leg: str, est_s: float) -> bool:
+        """Run a leg only if its estimated time still fits the run's budget."""
+        left = args.time_budget - (time.perf_counter() - T_START)
+        if est_s <= left:
+            return True
+        skipped[leg] = f"estimated {est_s:.0f} s, {left:.0f} s of the {args.time_budget:.0f} s budget left"
+        print(f"[bench] skipping {leg}: {skipped[leg]}", file=sys.stderr, flush=True)
+        return False
+
+    cpu = None
+    if not args.no_cpu and fits("sweep cpu_baseline", 75.0):
+        cpu = cpu_baseline(n, gates, 3 * n - 1, m)   # the H layer and the first full layer
+    fusion = None
+    if not args.no_fusion and n == 30 and fits("fusion", 110.0):
+        cases = [(int(c.split(":")[0]), m.C64 if c.split(":")[1] == "c64" else m.C128)
+                 for c in args.fusion_cases.split(",")]
+        fusion = {"workload": f"random SU(2) per qubit + CZ brickwork, depth {args.fusion_depth}, fused into <= 5-qubit "
+                              "blocks composed in FP64, contracted on MFMA (FP32 for complex64, FP64 for complex128)",
+                  "cases": run_fusion_workload(m, args.fusion_depth, cases)}
+        if not args.no_cpu and fits("fusion cpu_baseline", 45.0):
+            n0 = cases[0][0]
+            fusion["cpu_baseline"] = fusion_cpu_baseline(n0, workloads.su2_cz_circuit(n0, args.fusion_depth, SEED + n0),
+                                                         10.0)
     batch = None
-    if not args.no_batch:
-        batch = run_batch_workload(m, args.batch_count, args.batch_steps, 1, 15.0, not args.no_cpu)
+    if not args.no_batch and fits("batch", 40.0):
+        batch = run_batch_workload(m, args.batch_count, args.batch_steps, 1, 10.0, not args.no_cpu,
+                                   args.batch_plain_sample)
     serving = None
-    if not args.no_serve:
+    if not args.no_serve and fits("serving", 80.0):
         served = run_serving_workload(m, args.serve_count, args.serve_rate, False)
         plain = run_serving_workload(m, args.serve_count, args.serve_rate, True)
         cpu_serve = None
         if not args.no_cpu:
             cpu_serve = serving_cpu_baseline(workloads.serving_requests(args.serve_count, SEED + 5, args.serve_rate),
-                                             15.0)
+                                             10.0)
         # the same request stream at other arrival rates, served only: plain
         # dispatch runs one request at a time, so its circuits/s does not
         # depend on the arrival rate once arrivals outpace it (~9/s)
         by_rate = {}
         for r in [float(x) for x in args.serve_sweep.split(",") if x.strip()]:
-            if r == args.serve_rate:
+            if r == args.serve_rate or not fits(f"serving at {r:g}/s", 12.0):
                 continue
             sv = run_serving_workload(m, args.serve_count, r, False)
             by_rate[f"{r:g}"] = {"circuits_per_s": sv["circuits_per_s"], "p50_ms": sv["p50_ms"],
@@ -674,8 +794,11 @@ def main() -> int:
                      "traffic": traffic},
         "cpu_baseline": cpu,
         "norm_error": abs(nrm - 1.0),
+        "fusion": fusion,
         "batch": batch,
         "serving": serving,
+        "skipped_for_time": skipped or None,
+        "bench_seconds": round(time.perf_counter() - T_START, 1),
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -84,6 +84,14 @@ MOONLAB_API int moonlab_mi355x_set_state_cache_bytes(uint64_t bytes);
 
 typedef struct moonlab_mi355x_batch moonlab_mi355x_batch_t;
 
+/* One Hamiltonian term, the reference's pauli_term_t (vqe.h:101-105):
+ * pauli_string[q] in "IXYZ" is the operator on qubit q, exactly num_qubits
+ * characters; Y = i X Z. */
+typedef struct {
+    double      coefficient;
+    const char *pauli_string;
+} moonlab_mi355x_pauli_term_t;
+
 #define MOONLAB_MI355X_OBS_Z  1   /* <Z_q> for every qubit (and the norm) */
 #define MOONLAB_MI355X_OBS_ZZ 2   /* <Z_q Z_q+1>, q = 0..n-2 */
 #define MOONLAB_MI355X_OBS_X  4   /* <X_q> for every qubit */
@@ -104,6 +112,12 @@ typedef struct {
     double                   *expect_x;     /* n values */
     uint64_t                 *outcomes;     /* num_shots values, the collapse rule of
                                                measurement_all_qubits (measurement.c:263-307) */
+    /* Pauli-sum observables (vqe_compute_energy's terms, vqe.c:1287-1350):
+     * <P_k> for each string, measured exactly on the final state */
+    int                                num_pauli_terms;
+    const moonlab_mi355x_pauli_term_t *pauli_terms;
+    double                            *pauli_expect;   /* num_pauli_terms values, or NULL */
+    double                            *pauli_energy;   /* sum_k c_k <P_k> in term order, or NULL */
 } moonlab_mi355x_batch_item_t;
 
 typedef struct {
@@ -165,6 +179,62 @@ MOONLAB_API int moonlab_mi355x_hea_tfim_gradients(moonlab_mi355x_batch_t *b,
                                                   const moonlab_mi355x_hea_t *instances, int count,
                                                   int precision, double *energies, double *gradients,
                                                   moonlab_mi355x_batch_stats_t *stats);
+
+/* ---- Variational workloads over Pauli-sum Hamiltonians ---------------------
+ *
+ * The reference's VQE solver (src/algorithms/vqe.h:545-823) evaluates one
+ * parameter set at a time on the CPU.  These calls take many instances and
+ * run every circuit they need -- base points, +-pi/2 shifts, the replays with
+ * a parameter's generator inserted -- in ONE persistent batch launch.
+ *
+ * An instance is the reference's hardware-efficient ansatz
+ * (vqe.c:764-781): X on the Hartree-Fock bits (vqe.c:1302-1305), then per
+ * layer RY(theta[p]), RZ(theta[p+1]) on every qubit and an entangling ladder
+ * over (q, q+1) -- CNOT with control q (MOONLAB_MI355X_ENT_CNOT, the
+ * reference's) or CZ (MOONLAB_MI355X_ENT_CZ); angles are [layer][qubit][ry, rz],
+ * P = 2 n L of them.  E = sum_k c_k <P_k> + constant (vqe_compute_energy,
+ * vqe.c:1287-1350; the constant is nuclear_repulsion). */
+#define MOONLAB_MI355X_ENT_CZ   0
+#define MOONLAB_MI355X_ENT_CNOT 1
+
+typedef struct {
+    int                                num_qubits;     /* 1..24 */
+    int                                num_layers;
+    const double                      *angles;         /* 2 n L values */
+    int                                entangler;      /* MOONLAB_MI355X_ENT_* */
+    uint64_t                           hf_reference;   /* bit q = 1: X on qubit q first */
+    int                                num_terms;
+    const moonlab_mi355x_pauli_term_t *terms;          /* strings of num_qubits characters */
+    double                             constant;
+} moonlab_mi355x_vqe_t;
+
+/* Energies, one per instance. */
+MOONLAB_API int moonlab_mi355x_vqe_energies(moonlab_mi355x_batch_t *b, const moonlab_mi355x_vqe_t *instances,
+                                            int count, int precision, double *energies,
+                                            moonlab_mi355x_batch_stats_t *stats);
+/* Energies and parameter-shift gradients (vqe_compute_gradient, vqe.c:2094-2130):
+ * gradients holds sum_i P_i values, instance after instance. */
+MOONLAB_API int moonlab_mi355x_vqe_gradients(moonlab_mi355x_batch_t *b, const moonlab_mi355x_vqe_t *instances,
+                                             int count, int precision, double *energies, double *gradients,
+                                             moonlab_mi355x_batch_stats_t *stats);
+/* Quantum geometric tensor Q_ij = <d_i psi|d_j psi> - <d_i psi|psi><psi|d_j psi>
+ * with exact derivatives (the generator inserted after its rotation,
+ * vqe_qng.c:66-170), complex128: qgt = Re Q (vqe_compute_qgt) and berry =
+ * -2 Im Q (vqe_compute_berry_curvature), each P_i x P_i row-major, instance
+ * after instance; either may be NULL.  The P_i + 1 states of an instance
+ * are held on the device at once (MOONLAB_SCHED_OOM past 16 GiB). */
+MOONLAB_API int moonlab_mi355x_vqe_qgt(moonlab_mi355x_batch_t *b, const moonlab_mi355x_vqe_t *instances, int count,
+                                       double *qgt, double *berry, moonlab_mi355x_batch_stats_t *stats);
+/* Natural-gradient directions (g + regularization I)^-1 gradient per
+ * instance (vqe_natural_gradient_direction, vqe_qng.c:264-282), gradients /
+ * directions laid out as in moonlab_mi355x_vqe_gradients.  status (may be
+ * NULL) gets 0, or -1 where the metric is singular -- that instance's
+ * direction is then its gradient, the fallback the reference asks callers
+ * to take. */
+MOONLAB_API int moonlab_mi355x_vqe_natural_gradient(moonlab_mi355x_batch_t *b, const moonlab_mi355x_vqe_t *instances,
+                                                    int count, const double *gradients, double regularization,
+                                                    double *directions, int *status,
+                                                    moonlab_mi355x_batch_stats_t *stats);
 
 /* ---- Serving: continuous batching with HBM-footprint admission --------------
  *

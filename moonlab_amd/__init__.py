@@ -155,7 +155,36 @@ class _BatchItem(ctypes.Structure):
                 ("expect_z", ctypes.POINTER(ctypes.c_double)),
                 ("zz_chain", ctypes.POINTER(ctypes.c_double)),
                 ("expect_x", ctypes.POINTER(ctypes.c_double)),
-                ("outcomes", ctypes.POINTER(ctypes.c_uint64))]
+                ("outcomes", ctypes.POINTER(ctypes.c_uint64)),
+                ("num_pauli_terms", ctypes.c_int),
+                ("pauli_terms", ctypes.c_void_p),
+                ("pauli_expect", ctypes.POINTER(ctypes.c_double)),
+                ("pauli_energy", ctypes.POINTER(ctypes.c_double))]
+
+
+class _PauliTerm(ctypes.Structure):   # moonlab_mi355x_pauli_term_t (pauli_term_t, vqe.h:101-105)
+    _fields_ = [("coefficient", ctypes.c_double), ("pauli_string", ctypes.c_char_p)]
+
+
+class _Vqe(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_int), ("num_layers", ctypes.c_int),
+                ("angles", ctypes.POINTER(ctypes.c_double)), ("entangler", ctypes.c_int),
+                ("hf_reference", ctypes.c_uint64), ("num_terms", ctypes.c_int),
+                ("terms", ctypes.POINTER(_PauliTerm)), ("constant", ctypes.c_double)]
+
+
+ENT_CZ, ENT_CNOT = 0, 1
+
+
+def _pauli_terms(terms):
+    """(coefficient, string) pairs -> a moonlab_mi355x_pauli_term_t array and
+    the objects it points into."""
+    arr = (_PauliTerm * max(len(terms), 1))()
+    strs = [t[1].encode() for t in terms]
+    for i, (c, _) in enumerate(terms):
+        arr[i].coefficient = float(c)
+        arr[i].pauli_string = strs[i]
+    return arr, strs
 
 
 class _BatchStats(ctypes.Structure):
@@ -285,6 +314,12 @@ def _declare(L: ctypes.CDLL) -> None:
         "moonlab_mi355x_hea_tfim_energies": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
         "moonlab_mi355x_hea_tfim_gradients": ([vp, ctypes.POINTER(_Hea), ip, ip, dp, dp,
                                                ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_vqe_energies": ([vp, ctypes.POINTER(_Vqe), ip, ip, dp, ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_vqe_gradients": ([vp, ctypes.POINTER(_Vqe), ip, ip, dp, dp, ctypes.POINTER(_BatchStats)],
+                                         ip),
+        "moonlab_mi355x_vqe_qgt": ([vp, ctypes.POINTER(_Vqe), ip, dp, dp, ctypes.POINTER(_BatchStats)], ip),
+        "moonlab_mi355x_vqe_natural_gradient": ([vp, ctypes.POINTER(_Vqe), ip, dp, ctypes.c_double, dp,
+                                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_BatchStats)], ip),
         # state-vector layer
         "moonlab_hip_device_count": ([], ip),
         "moonlab_hip_last_error": ([], ctypes.c_char_p),
@@ -917,6 +952,8 @@ class BatchResult:
     zz_chain: Optional[np.ndarray]
     expect_x: Optional[np.ndarray]
     outcomes: Optional[np.ndarray]
+    pauli_expect: Optional[np.ndarray] = None
+    pauli_energy: Optional[float] = None
 
 
 class Batch:
@@ -980,13 +1017,115 @@ class Batch:
             it.zz_chain = dp(zz) if obs & OBS_ZZ else None
             it.expect_x = dp(x) if obs & OBS_X else None
             it.outcomes = oc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)) if draws.size else None
+            terms = c.get("pauli", [])
+            pe = np.zeros(max(len(terms), 1))
+            pen = np.zeros(1)
+            if terms:
+                parr, pstr = _pauli_terms(terms)
+                keep += [parr, pstr, pe, pen]
+                it.num_pauli_terms = len(terms)
+                it.pauli_terms = ctypes.cast(parr, ctypes.c_void_p)
+                it.pauli_expect = dp(pe)
+                it.pauli_energy = dp(pen)
             outs.append((norm, z if obs & OBS_Z else None, zz[:max(n - 1, 0)] if obs & OBS_ZZ else None,
-                         x if obs & OBS_X else None, oc[:draws.size] if draws.size else None))
+                         x if obs & OBS_X else None, oc[:draws.size] if draws.size else None,
+                         pe[:len(terms)] if terms else None, pen if terms else None))
         st = _BatchStats()
         fn = lib().moonlab_mi355x_batch_autotune if autotune else lib().moonlab_mi355x_batch_run
         _check_sched(fn(self._h, items, len(circuits), ctypes.byref(st)), "batch_run")
         self.last_stats = self._stats(st)
-        return [BatchResult(float(o[0][0]), o[1], o[2], o[3], o[4]) for o in outs]
+        return [BatchResult(float(o[0][0]), o[1], o[2], o[3], o[4], o[5],
+                            float(o[6][0]) if o[6] is not None else None) for o in outs]
+
+    @staticmethod
+    def _vqe_array(instances):
+        """instances: dicts with n, layers, angles ([layers, n, 2] or flat),
+        terms ((coefficient, string) pairs), entangler (ENT_CNOT default: the
+        reference ansatz), hf (Hartree-Fock bits), constant."""
+        k = len(instances)
+        arr = (_Vqe * max(k, 1))()
+        keep = []
+        for i, v in enumerate(instances):
+            n, L = int(v["n"]), int(v["layers"])
+            a = np.ascontiguousarray(v["angles"], dtype=np.float64).reshape(-1)
+            if a.size != 2 * n * L:
+                raise ValueError("angles must hold layers x qubits x 2 values")
+            terms = v.get("terms", [])
+            parr, pstr = _pauli_terms(terms)
+            keep += [a, parr, pstr]
+            arr[i].num_qubits = n
+            arr[i].num_layers = L
+            arr[i].angles = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            arr[i].entangler = int(v.get("entangler", ENT_CNOT))
+            arr[i].hf_reference = int(v.get("hf", 0))
+            arr[i].num_terms = len(terms)
+            arr[i].terms = ctypes.cast(parr, ctypes.POINTER(_PauliTerm))
+            arr[i].constant = float(v.get("constant", 0.0))
+        return arr, keep
+
+    def vqe_energies(self, instances, precision: int = C128) -> np.ndarray:
+        arr, keep = self._vqe_array(instances)
+        e = np.zeros(max(len(instances), 1))
+        st = _BatchStats()
+        _check_sched(lib().moonlab_mi355x_vqe_energies(self._h, arr, len(instances), precision,
+                                                       e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                       ctypes.byref(st)), "vqe_energies")
+        self.last_stats = self._stats(st)
+        del keep
+        return e[:len(instances)]
+
+    def vqe_gradients(self, instances, precision: int = C128):
+        """(energies, [gradient of P = 2 n L values per instance])."""
+        arr, keep = self._vqe_array(instances)
+        sizes = [2 * int(v["n"]) * int(v["layers"]) for v in instances]
+        e = np.zeros(max(len(instances), 1))
+        g = np.zeros(max(sum(sizes), 1))
+        st = _BatchStats()
+        dp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        _check_sched(lib().moonlab_mi355x_vqe_gradients(self._h, arr, len(instances), precision, dp(e), dp(g),
+                                                        ctypes.byref(st)), "vqe_gradients")
+        self.last_stats = self._stats(st)
+        del keep
+        offs = np.cumsum([0] + sizes)
+        return e[:len(instances)], [g[offs[i]:offs[i + 1]].copy() for i in range(len(instances))]
+
+    def vqe_qgt(self, instances):
+        """[(metric, Berry curvature)] per instance, each P x P, complex128 circuits."""
+        arr, keep = self._vqe_array(instances)
+        sizes = [(2 * int(v["n"]) * int(v["layers"])) ** 2 for v in instances]
+        q = np.zeros(max(sum(sizes), 1))
+        f = np.zeros(max(sum(sizes), 1))
+        st = _BatchStats()
+        dp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        _check_sched(lib().moonlab_mi355x_vqe_qgt(self._h, arr, len(instances), dp(q), dp(f), ctypes.byref(st)),
+                     "vqe_qgt")
+        self.last_stats = self._stats(st)
+        del keep
+        offs = np.cumsum([0] + sizes)
+        out = []
+        for i, v in enumerate(instances):
+            P = 2 * int(v["n"]) * int(v["layers"])
+            out.append((q[offs[i]:offs[i + 1]].reshape(P, P).copy(), f[offs[i]:offs[i + 1]].reshape(P, P).copy()))
+        return out
+
+    def vqe_natural_gradient(self, instances, gradients, regularization: float = 1e-3):
+        """[(direction, status)] per instance: (g + eps I)^-1 gradient, status
+        -1 (direction = gradient) where the metric is singular."""
+        arr, keep = self._vqe_array(instances)
+        g = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in gradients])
+                                 if len(gradients) else np.zeros(1))
+        d = np.zeros_like(g)
+        stt = np.zeros(max(len(instances), 1), dtype=np.int32)
+        st = _BatchStats()
+        dp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        _check_sched(lib().moonlab_mi355x_vqe_natural_gradient(
+            self._h, arr, len(instances), dp(g), regularization, dp(d),
+            stt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(st)), "vqe_natural_gradient")
+        self.last_stats = self._stats(st)
+        del keep
+        sizes = [2 * int(v["n"]) * int(v["layers"]) for v in instances]
+        offs = np.cumsum([0] + sizes)
+        return [(d[offs[i]:offs[i + 1]].copy(), int(stt[i])) for i in range(len(instances))]
 
     @staticmethod
     def _hea_array(instances):

@@ -34,8 +34,12 @@ BUILD = os.path.join(PKG, "_build")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libmoonlab_mi355x.so")
 PLUGIN = os.path.join(LIBDIR, "libmoonlab_mi355x_plugin.so")
+ROUTE = os.path.join(LIBDIR, "libmoonlab_mi355x_route.so")
 # translation units that implement the job / circuit API (left out of the plugin)
 HOST_API = ("scheduler.cpp", "qgtl_circuit.cpp", "control_frame.cpp")
+# the reference's GPU seam (quantum_state_create_gpu, qsim_gpu_route_*): its own
+# library over the plugin, so only a process that links it gets those symbols
+ROUTE_SRC = "gpu_route.cpp"
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -78,16 +82,22 @@ def build(verbose: bool = True) -> str:
     workers = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(_compile, srcs))
+    route_obj = [o for o in objs if os.path.basename(o)[:-2] == ROUTE_SRC]
+    objs = [o for o in objs if os.path.basename(o)[:-2] != ROUTE_SRC]
     plugin_objs = [o for o in objs if not os.path.basename(o)[:-2] in HOST_API]
-    for out, parts in ((LIB, objs), (PLUGIN, plugin_objs)):
+    links = ((LIB, objs, []), (PLUGIN, plugin_objs, []),
+             (ROUTE, route_obj, ["-L" + LIBDIR, "-lmoonlab_mi355x_plugin", "-Wl,-rpath,$ORIGIN"]))
+    for out, parts, extra in links:
         tmp = out + ".tmp"
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", tmp] + parts + ["-lpthread"]
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", tmp] + parts + extra + \
+              ["-lpthread"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"link of {os.path.basename(out)} failed:\n{r.stderr}")
         os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources and {os.path.basename(PLUGIN)} from {len(plugin_objs)}")
+        print(f"built {LIB} from {len(objs)} sources, {os.path.basename(PLUGIN)} from {len(plugin_objs)} and "
+              f"{os.path.basename(ROUTE)}")
     return LIB
 
 

@@ -43,8 +43,33 @@ hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BO
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
-                        uint64_t tile_slot, const double *draws, double *obs, uint64_t *outcomes, int grid,
-                        hipStream_t st);
+                        uint64_t tile_slot, const double *draws, double *obs, uint64_t *outcomes,
+                        const PauliT *terms, void *state_out, int grid, hipStream_t st);
+
+// A Pauli string over n qubits (s[q] in "IXYZ" acts on qubit q, strlen == n,
+// pauli_term_t of vqe.h:101-105) as masks; false on any other character or
+// length.
+bool parse_pauli(const char *s, int n, PauliT &out)
+{
+    if (!s) return false;
+    out = PauliT{};
+    int len = 0;
+    for (; s[len]; len++) {
+        if (len >= n) return false;
+        switch (s[len]) {
+        case 'I': break;
+        case 'X': out.x |= 1ull << len; break;
+        case 'Z': out.z |= 1ull << len; break;
+        case 'Y':
+            out.x |= 1ull << len;
+            out.z |= 1ull << len;
+            out.ny++;
+            break;
+        default: return false;
+        }
+    }
+    return len == n;
+}
 
 namespace {
 
@@ -633,6 +658,9 @@ struct moonlab_mi355x_batch {
     hipGraphExec_t  gexec = nullptr;
     std::vector<uint64_t> gkey;       // shape of gexec
     std::vector<uint64_t> last_key;   // shape of the previous run
+    // state and overlap buffers of the geometric-tensor calls (vqe.cpp)
+    void       *d_states = nullptr;
+    size_t      states_cap = 0;
 };
 
 namespace {
@@ -685,7 +713,7 @@ int host_threads(int work_items)
 }
 
 int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int count,
-              moonlab_mi355x_batch_stats_t *stats)
+              moonlab_mi355x_batch_stats_t *stats, const BatchStateOut *state_out = nullptr)
 {
     const double t0 = now_s();
     if (!b || count < 0 || (count > 0 && !items)) return MOONLAB_SCHED_BAD_ARG;
@@ -693,6 +721,10 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     const int tbmax = b->tile_bits;
     std::vector<BInst> insts((size_t)count);
     std::vector<double> draws;
+    std::vector<PauliT> pauli;
+    for (int i = 0; i < count; i++)
+        if (items[i].num_pauli_terms < 0 || (items[i].num_pauli_terms > 0 && !items[i].pauli_terms))
+            return MOONLAB_SCHED_BAD_ARG;
     std::vector<std::pair<double, int>> cost((size_t)count);
     int64_t obs_total = 0, out_total = 0;
     int max_big = 0;
@@ -774,7 +806,15 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         if (it.observables & MOONLAB_MI355X_OBS_X) I.flags |= BF_X;
         I.num_shots = it.num_shots;
         I.obs_offset = obs_total;
-        obs_total += obs_size(n);
+        I.num_pauli = it.num_pauli_terms > 0 ? it.num_pauli_terms : 0;
+        I.pauli_offset = (int64_t)pauli.size();
+        for (int k = 0; k < I.num_pauli; k++) {
+            PauliT pt;
+            if (!parse_pauli(it.pauli_terms[k].pauli_string, n, pt)) return MOONLAB_SCHED_BAD_ARG;
+            pauli.push_back(pt);
+        }
+        I.state_offset = state_out ? state_out->offsets[i] : -1;
+        obs_total += obs_size(n) + I.num_pauli;
         I.draw_offset = (int64_t)draws.size();
         for (int sh = 0; sh < it.num_shots; sh++) draws.push_back(it.draws[sh]);
         I.out_offset = out_total;
@@ -811,6 +851,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
     const size_t o_pass = place((total_passes + 1) * sizeof(BPass));
     const size_t o_ops = place((total_ops + 1) * sizeof(BOp));
     const size_t o_draw = place((draws.size() + 1) * sizeof(double));
+    const size_t o_pauli = place((pauli.size() + 1) * sizeof(PauliT));
     const size_t prog_bytes = off;
     off = 0;
     const size_t r_obs = place((size_t)(obs_total + 1) * sizeof(double));
@@ -837,6 +878,8 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         on_threads(nthreads, place_chunk);
     }
     if (!draws.empty()) memcpy(hp + o_draw, draws.data(), draws.size() * sizeof(double));
+    if (!pauli.empty()) memcpy(hp + o_pauli, pauli.data(), pauli.size() * sizeof(PauliT));
+    void *d_state = state_out ? state_out->d_buf : nullptr;
     char *dp = (char *)b->d_prog;
     char *dr = (char *)b->d_res;
     double2 *scratch = (double2 *)b->d_scratch;
@@ -849,7 +892,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
             r = launch_batch((const BInst *)(dp + o_inst), (const int32_t *)(dp + o_order), count, b->d_counter,
                              (const BPass *)(dp + o_pass), (const BOp *)(dp + o_ops), scratch, slot_amps,
                              tile_scratch, tile_slot, (const double *)(dp + o_draw), (double *)(dr + r_obs),
-                             (uint64_t *)(dr + r_out), grid, b->stream);
+                             (uint64_t *)(dr + r_out), (const PauliT *)(dp + o_pauli), d_state, grid, b->stream);
         if (r == hipSuccess) r = hipMemcpyAsync(b->h_res, dr, res_bytes, hipMemcpyDeviceToHost, b->stream);
         return r;
     };
@@ -858,7 +901,7 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
                                        (uint64_t)(uintptr_t)hp, (uint64_t)(uintptr_t)b->h_res,
                                        (uint64_t)(uintptr_t)b->d_scratch, prog_bytes, res_bytes, o_inst, o_order,
                                        o_pass, o_ops, o_draw, r_obs, r_out, (uint64_t)count, (uint64_t)grid,
-                                       slot_amps, tile_slot};
+                                       slot_amps, tile_slot, o_pauli, (uint64_t)(uintptr_t)d_state};
     // A shape seen for the first time runs eagerly; its second run is captured
     // and launched as a graph; later runs of it replay that graph.
     bool replayed = false;
@@ -917,6 +960,15 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
         if (it.expect_x)
             for (int q = 0; q < n; q++) it.expect_x[q] = o[1 + n + (n > 1 ? n - 1 : 0) + q];
         if (it.num_shots > 0) memcpy(it.outcomes, outc + I.out_offset, (size_t)it.num_shots * sizeof(uint64_t));
+        if (I.num_pauli > 0) {
+            const double *pv = o + obs_size(n);
+            if (it.pauli_expect) memcpy(it.pauli_expect, pv, (size_t)I.num_pauli * sizeof(double));
+            if (it.pauli_energy) {   // sum_k c_k <P_k> in term order (vqe.c:1326-1342)
+                double e = 0.0;
+                for (int k = 0; k < I.num_pauli; k++) e += it.pauli_terms[k].coefficient * pv[k];
+                *it.pauli_energy = e;
+            }
+        }
     }
     if (stats) {
         float ms = 0.f;
@@ -937,6 +989,21 @@ int run_batch(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int
 }
 
 }  // namespace
+
+// Entry points of the batch engine for the variational calls (vqe.cpp).
+int batch_run_states(moonlab_mi355x_batch_t *b, moonlab_mi355x_batch_item_t *items, int count,
+                     moonlab_mi355x_batch_stats_t *stats, const BatchStateOut *out)
+{
+    return run_batch(b, items, count, stats, out);
+}
+
+void *batch_state_buffer(moonlab_mi355x_batch_t *b, size_t bytes)
+{
+    if (grow_dev(&b->d_states, &b->states_cap, bytes)) return nullptr;
+    return b->d_states;
+}
+
+hipStream_t batch_stream(moonlab_mi355x_batch_t *b) { return b->stream; }
 
 extern "C" {
 
@@ -1041,6 +1108,7 @@ void moonlab_mi355x_batch_free(moonlab_mi355x_batch_t *b)
     hipFree(b->d_res);
     hipFree(b->d_counter);
     hipFree(b->d_scratch);
+    hipFree(b->d_states);
     if (b->h_prog) hipHostFree(b->h_prog);
     if (b->h_res) hipHostFree(b->h_res);
     if (b->gexec) hipGraphExecDestroy(b->gexec);

@@ -398,6 +398,47 @@ __device__ void measure_tile(Shared &sh, const cplx_t<R> *t, int T, int tb, int 
     }
 }
 
+// Pauli strings over one tile of the low tb qubits (tiles in index order):
+// term k adds sum_l Re[conj(a_(i^x)) a_i i^ny (-1)^popcount(i & z)], i =
+// base | l, into out[k].  The partner i ^ x is in the tile when x has no bit
+// at or above tb; otherwise it is read from the circuit's HBM slot (the
+// final state, which measurement does not modify).  Terms are reduced eight
+// at a time; every sum runs in a fixed order, and out[k] is updated by one
+// thread, tile after tile.
+template <typename R>
+__device__ void measure_pauli_tile(Shared &sh, const cplx_t<R> *t, int T, uint64_t base, const cplx_t<R> *slot,
+                                   const PauliT *terms, int nterms, double *out)
+{
+    const int tid = threadIdx.x;
+    const uint64_t lowmask = (uint64_t)T - 1ull;
+    for (int c0 = 0; c0 < nterms; c0 += 8) {
+        const int cnt = nterms - c0 < 8 ? nterms - c0 : 8;
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            v[j] = 0.0;
+            if (j >= cnt) continue;   // uniform across the block
+            const uint64_t x = terms[c0 + j].x, z = terms[c0 + j].z;
+            const int ny = terms[c0 + j].ny & 3;
+            const uint64_t xl = x & lowmask, xh = x & ~lowmask;
+            double s = 0.0;
+            for (int l = tid; l < T; l += NT) {
+                const cplx_t<R> a = t[l];
+                const cplx_t<R> b = xh == 0 ? t[(uint64_t)l ^ xl] : slot[(base ^ xh) | ((uint64_t)l ^ xl)];
+                const double br = (double)b.x, bi = (double)b.y, ar = (double)a.x, ai = (double)a.y;
+                const double re = fmar(br, ar, bi * ai);     // Re conj(b) a
+                const double im = fmar(br, ai, -(bi * ar));  // Im conj(b) a
+                const double c = ny == 0 ? re : ny == 1 ? -im : ny == 2 ? -re : im;
+                s += (__popcll((base | (uint64_t)l) & z) & 1) ? -c : c;
+            }
+            v[j] = s;
+        }
+        block_reduce(sh, v, cnt, sh.res);
+        if (tid < cnt) out[c0 + tid] += sh.res[tid];
+        __syncthreads();
+    }
+}
+
 // Inclusive scan of p over T entries in LDS (as doubles), index order.
 __device__ void lds_scan(Shared &sh, double *cdf, int T, double offset)
 {
@@ -430,7 +471,7 @@ __device__ void lds_scan(Shared &sh, double *cdf, int T, double offset)
 template <typename R>
 __device__ void run_circuit(Shared &sh, const BInst &I, const BPass *passes, const BOp *ops,
                             double2 *slot_raw, double *tile_sums, const double *draws, double *obs,
-                            uint64_t *outcomes)
+                            uint64_t *outcomes, const PauliT *terms, void *state_out)
 {
     const int tid = threadIdx.x;
     const int n = I.n, tb = I.tile_bits, T = 1 << tb;
@@ -471,8 +512,23 @@ __device__ void run_circuit(Shared &sh, const BInst &I, const BPass *passes, con
         }
     }
 
+    // ---- the final state, when asked for (geometric-tensor overlaps) ----
+    if (I.state_offset >= 0) {
+        cplx_t<R> *so = (cplx_t<R> *)state_out + I.state_offset;
+        if (resident) {
+            for (int l = tid; l < T; l += NT) so[l] = t[l];
+        } else {
+            const uint64_t dim = 1ull << n;
+            for (uint64_t i = tid; i < dim; i += NT) so[i] = slot[i];
+        }
+        __syncthreads();
+    }
+
     // ---- measurement pass 0: tiles of the low tb qubits, index order ----
     const bool want_shots = I.num_shots > 0;
+    double *pauli_out = obs + I.obs_offset + obs_size(n);
+    for (int k = tid; k < I.num_pauli; k += NT) pauli_out[k] = 0.0;
+    __syncthreads();
     for (uint64_t ti = 0; ti < ntiles; ti++) {
         const uint64_t base = ti << tb;
         if (!resident) {
@@ -483,6 +539,8 @@ __device__ void run_circuit(Shared &sh, const BInst &I, const BPass *passes, con
         measure_tile<R>(sh, t, T, tb, n, base, I.flags, 0, &tt);
         if (!resident && tid == 0 && want_shots) tile_sums[ti] = tt;
         __syncthreads();
+        if (I.num_pauli > 0)
+            measure_pauli_tile<R>(sh, t, T, base, slot, terms + I.pauli_offset, I.num_pauli, pauli_out);
     }
     // ---- measurement pass 1: X on the qubits above the low tile ----
     if ((I.flags & BF_X) && !resident) {
@@ -611,7 +669,8 @@ __global__ __launch_bounds__(NT) void k_batch(const BInst *__restrict__ insts,
                                               const BOp *__restrict__ ops, double2 *__restrict__ scratch,
                                               uint64_t slot_amps, double *__restrict__ tile_scratch,
                                               uint64_t tile_slot, const double *__restrict__ draws,
-                                              double *__restrict__ obs, uint64_t *__restrict__ outcomes)
+                                              double *__restrict__ obs, uint64_t *__restrict__ outcomes,
+                                              const PauliT *__restrict__ terms, void *__restrict__ state_out)
 {
     __shared__ Shared sh;
     double2 *slot = scratch + (uint64_t)blockIdx.x * slot_amps;
@@ -627,9 +686,9 @@ __global__ __launch_bounds__(NT) void k_batch(const BInst *__restrict__ insts,
         if (k >= ninst) break;
         const BInst I = insts[order[k]];
         if (I.prec == 1)
-            run_circuit<float>(sh, I, passes, ops, slot, ts, draws, obs, outcomes);
+            run_circuit<float>(sh, I, passes, ops, slot, ts, draws, obs, outcomes, terms, state_out);
         else
-            run_circuit<double>(sh, I, passes, ops, slot, ts, draws, obs, outcomes);
+            run_circuit<double>(sh, I, passes, ops, slot, ts, draws, obs, outcomes, terms, state_out);
         __syncthreads();
     }
 }
@@ -745,12 +804,12 @@ hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const B
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
-                        uint64_t tile_slot, const double *draws, double *obs, uint64_t *outcomes, int grid,
-                        hipStream_t st)
+                        uint64_t tile_slot, const double *draws, double *obs, uint64_t *outcomes,
+                        const PauliT *terms, void *state_out, int grid, hipStream_t st)
 {
     k_reset_counter<<<1, 1, 0, st>>>(counter);
     k_batch<<<grid, NT, 0, st>>>(insts, order, ninst, counter, passes, ops, scratch, slot_amps, tile_scratch,
-                                 tile_slot, draws, obs, outcomes);
+                                 tile_slot, draws, obs, outcomes, terms, state_out);
     return hipGetLastError();
 }
 

@@ -142,6 +142,20 @@ struct BInst {
     int64_t  obs_offset;    // into the FP64 observable buffer
     int64_t  draw_offset;   // into the uniform-draw buffer
     int64_t  out_offset;    // into the outcome buffer
+    int64_t  pauli_offset;  // first Pauli term (PauliT array); results follow the standard observables
+    int64_t  state_offset;  // amplitude offset of the final state in the state output buffer, -1: none
+    int32_t  num_pauli;     // Pauli terms to measure
+    int32_t  pad_;
+};
+
+// One Pauli string P = i^ny X^x Z^z (Y_q = i X_q Z_q): P|i> = i^ny
+// (-1)^popcount(i & z) |i ^ x>, so <P> = sum_i Re[conj(a_(i^x)) a_i i^ny
+// (-1)^popcount(i & z)].
+struct PauliT {
+    uint64_t x;
+    uint64_t z;
+    int32_t  ny;            // number of Y factors
+    int32_t  pad;
 };
 
 enum : int32_t {
@@ -151,7 +165,7 @@ enum : int32_t {
 };
 
 // Observable block per instance, FP64: [norm, P1[n], ZZ[n-1], X[n]].
-inline int64_t obs_size(int n) { return 1 + n + (n > 1 ? n - 1 : 0) + n; }
+__host__ __device__ inline int64_t obs_size(int n) { return 1 + n + (n > 1 ? n - 1 : 0) + n; }
 
 constexpr int kBatchMaxTileBits = 12;
 constexpr int kBatchThreads = 1024;

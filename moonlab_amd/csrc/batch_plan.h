@@ -48,4 +48,16 @@ void plan_passes(const std::vector<GOp> &gops, int n, int tb, int seg_regs, std:
 // the staged loop refuse to advance.
 bool segments_fit_staging(const BOp *ops, int count);
 
+// Pauli string (s[q] in "IXYZ" acts on qubit q, exactly n characters) as
+// masks; false otherwise.
+bool parse_pauli(const char *s, int n, PauliT &out);
+
+// Final states of a batch run written to device memory: item i's state (its
+// precision's amplitudes) lands at amplitude offset offsets[i] of d_buf, or
+// nowhere when offsets[i] < 0.
+struct BatchStateOut {
+    void          *d_buf;
+    const int64_t *offsets;
+};
+
 }  // namespace mlx

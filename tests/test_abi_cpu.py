@@ -17,10 +17,13 @@ from conftest import GOLDEN, ROOT
 INCLUDE = os.path.join(ROOT, "include")
 
 
+ROUTE_HEADER = "moonlab_gpu_route.h"   # the reference's GPU seam: libmoonlab_mi355x_route.so
+
+
 def declared_symbols():
     names = set()
     for f in os.listdir(INCLUDE):
-        if not f.endswith(".h"):
+        if not f.endswith(".h") or f == ROUTE_HEADER:
             continue
         text = open(os.path.join(INCLUDE, f)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -41,6 +44,63 @@ def test_library_exports_every_declared_symbol():
                          text=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if ln.split()[-1].startswith("moonlab_")}
     assert exported == names, sorted(exported ^ names)
+
+
+def test_route_library_exports_the_seam():
+    """libmoonlab_mi355x_route.so exports exactly the seam that
+    include/moonlab_gpu_route.h declares (the symbols of the reference's
+    state_gpu.cu / gpu_route_cpu_unavailable.c), and the engine library none
+    of them."""
+    import subprocess
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(INCLUDE, ROUTE_HEADER)).read(), flags=re.S)
+    text = "\n".join(ln for ln in text.splitlines() if not ln.lstrip().startswith("#"))
+    names = {mm.group(1) for mm in re.finditer(r"MOONLAB_API\s+[^;{]*?\b(\w+)\s*\(", text)}
+    assert {"quantum_state_create_gpu", "qsim_gpu_route_cnot", "moonlab_cuda_state_free"} <= names
+    route = os.path.join(os.path.dirname(m.LIB_PATH), "libmoonlab_mi355x_route.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", route], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.split()[1] in ("T", "W")}
+    assert exported == names, sorted(exported ^ names)
+    main = subprocess.run(["nm", "-D", "--defined-only", m.LIB_PATH], capture_output=True, text=True).stdout
+    assert not names & {ln.split()[-1] for ln in main.splitlines()}
+
+
+def test_routed_reference_without_device():
+    """The reference built with the seam in place of its CPU stub
+    (oracle/Makefile ref_routed): the seam's strong definitions replace
+    state.c's weak ones, nothing is left undefined, and with no GPU the
+    contract of the reference's tests/unit/test_gpu_sync_contract.c holds:
+    quantum_state_create_gpu fails loudly (NOT_SUPPORTED, NULL out), the sync
+    calls are no-ops on a CPU state and reject NULL."""
+    import subprocess
+    routed = os.path.join(ROOT, "oracle", "_ref", "libmoonlab_ref_routed.so")
+    if not os.path.exists(routed):
+        pytest.skip("oracle/_ref/libmoonlab_ref_routed.so not built (make -C oracle ref_routed)")
+    out = subprocess.run(["nm", "-D", routed], capture_output=True, text=True).stdout
+    kinds = {ln.split()[-1]: ln.split()[-2] for ln in out.splitlines() if len(ln.split()) >= 2}
+    for sym in ("quantum_state_create_gpu", "quantum_state_sync_to_host", "quantum_state_sync_from_host",
+                "qsim_gpu_route_apply_1q_matrix", "qsim_gpu_route_apply_2q_matrix", "moonlab_cuda_state_free"):
+        assert kinds.get(sym) == "T", (sym, kinds.get(sym))
+    undefined = subprocess.run(["nm", "-D", "--undefined-only", routed], capture_output=True, text=True).stdout
+    plugin = os.path.join(os.path.dirname(m.LIB_PATH), "libmoonlab_mi355x_plugin.so")
+    pdef = subprocess.run(["nm", "-D", "--defined-only", plugin], capture_output=True, text=True).stdout
+    provided = {ln.split()[-1] for ln in pdef.splitlines()}
+    left = [ln.split()[-1] for ln in undefined.splitlines()
+            if ("qsim_" in ln or "quantum_" in ln or "moonlab_" in ln) and " w " not in ln]
+    assert left and set(left) <= provided, sorted(set(left) - provided)   # the engine, from the plugin
+    if m.device_count() > 0:
+        pytest.skip("a GPU is visible: the device path is tested by tests/test_gpu_engine.py")
+    L = ctypes.CDLL(routed)
+    L.quantum_state_create_gpu.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+    p = ctypes.c_void_p(1)
+    assert L.quantum_state_create_gpu(4, ctypes.byref(p)) == -7 and not p.value
+    L.quantum_state_sync_to_host.argtypes = [ctypes.c_void_p]
+    assert L.quantum_state_sync_to_host(None) == -6
+    L.ref_state_new.restype = ctypes.c_void_p
+    L.ref_state_new.argtypes = [ctypes.c_int]
+    L.ref_state_delete.argtypes = [ctypes.c_void_p]
+    s = L.ref_state_new(3)
+    assert L.quantum_state_sync_to_host(s) == 0
+    L.ref_state_delete(s)
 
 
 def test_binding_declares_every_symbol_it_calls():
