@@ -580,9 +580,11 @@ def main() -> int:
     ap.add_argument("--time-budget", type=float, default=540.0,
                     help="seconds the whole default run may take: legs past it are skipped and named")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--workload", choices=("sweep", "fusion", "blocked"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "fusion", "blocked", "batch"), default="sweep",
                     help="sweep: the 30-qubit gate sweep (with the batch and serving legs); "
-                         "fusion: 32q complex64 and 33q complex128 SU(2)+CZ depth 60 on fused MFMA blocks")
+                         "fusion: 32q complex64 and 33q complex128 SU(2)+CZ depth 60 on fused MFMA blocks; "
+                         "blocked: cache-blocked register-phase passes; "
+                         "batch: the batch leg alone (for profiler runs)")
     ap.add_argument("--fusion-depth", type=int, default=60)
     ap.add_argument("--fusion-cases", default="32:c64,33:c128")
     ap.add_argument("--probe", action="store_true",
@@ -616,6 +618,19 @@ def main() -> int:
                          "unit": "GB/s", "frac": round(c128["register_TB_per_s_per_pass"] * 1e3 / HBM_PEAK_GBS, 4),
                          "traffic": None},
             "cases": res}), flush=True)
+        return 0
+
+    if args.workload == "batch":
+        t0 = time.perf_counter()
+        res = run_batch_workload(m, args.batch_count, args.batch_steps, 1, 10.0, not args.no_cpu,
+                                 args.batch_plain_sample)
+        wall = time.perf_counter() - t0
+        print(json.dumps({
+            "metric": "circuits/s, persistent batch launch (HEA + TFIM energy, complex128)",
+            "value": res["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1, "steps": args.batch_steps,
+            "warmup": 1, "ms_per_step": res["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": res["workload"]}, "batch": res, "bench_seconds": round(wall, 1)}), flush=True)
         return 0
 
     if args.workload == "fusion":
