@@ -23,15 +23,6 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# HIP maps streams onto a fixed set of hardware queues per process and runs
-# each queue in order.  The serving leg runs 2 batch engines and 2 large-
-# request workers on streams of their own, after the sweep and batch legs
-# have created theirs; with HIP's default of 4 queues two of those streams
-# can land on one queue and serialize.  8 queues keep them apart.  Set before
-# the HIP runtime starts (the first device call); a larger setting is kept.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
-
 import numpy as np  # noqa: E402
 
 T_START = time.perf_counter()

@@ -434,6 +434,19 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int 
                     b.pad = ARM_XPERM + b.lq;
                 }
             }
+            // the entries the arm reads go to m[0..3] (the register pass
+            // prefetches only those; the general arms read all eight)
+            double *m = b.m;
+            if (b.pad >= ARM_MAT1_REAL && b.pad < ARM_MAT1_REAL + 4) {
+                const double r[4] = {m[0], m[2], m[4], m[6]};   // u00, u01, u10, u11 (real)
+                memcpy(m, r, sizeof r);
+            } else if (b.pad >= ARM_MAT1_RX && b.pad < ARM_MAT1_RX + 4) {
+                const double r[4] = {m[0], m[3], m[5], m[6]};   // Re u00, Im u01, Im u10, Re u11
+                memcpy(m, r, sizeof r);
+            } else if (b.pad >= ARM_DIAG11 && b.pad < ARM_DIAG) {
+                const double r[2] = {m[6], m[7]};               // d3
+                memcpy(m, r, sizeof r);
+            }
             out.push_back(b);
         }
         rest.swap(later);

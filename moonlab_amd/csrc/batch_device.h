@@ -58,6 +58,10 @@ enum RegArm : int32_t {
     ARM_DIAG11 = 44,        // + 4 P + Q (P < Q): diag(1, 1, 1, d3), no controls
     ARM_DIAG = 60,          // any diagonal (kmap, kg, jm)
 };
+// The entries of m[] an arm reads: MAT1, MAT1_MASKED and DIAG all eight as
+// built; MAT1_REAL m[0..3] = (u00, u01, u10, u11); MAT1_RX m[0..3] = (Re u00,
+// Im u01, Im u10, Re u11); DIAG1 / DIAG1_ONLY1 m[0..3] = (d0, d1); DIAG11
+// m[0..1] = d3.
 
 // Register bits of a phase, ascending tile bits.
 struct RegLayout {

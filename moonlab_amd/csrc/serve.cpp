@@ -76,6 +76,7 @@ struct moonlab_mi355x_server {
     bool                             stopping = false;
     std::vector<std::thread>         threads;
     std::vector<moonlab_mi355x_batch_t *> engines;
+    std::vector<hipStream_t>         large_streams;   // one per large worker
     moonlab_mi355x_server_stats_t    stats{};
     double                           t0 = 0;
 };
@@ -182,12 +183,10 @@ void run_single(Req *r, int mode, hipStream_t stream)
     }
 }
 
-void large_thread(moonlab_mi355x_server_t *s)
+void large_thread(moonlab_mi355x_server_t *s, hipStream_t stream)
 {
     // one stream per worker for its whole life: a request's kernels queue
     // behind its own worker's only, not behind every pooled state's stream
-    hipStream_t stream = nullptr;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) stream = nullptr;
     for (;;) {
         Req *r = nullptr;
         {
@@ -214,7 +213,6 @@ void large_thread(moonlab_mi355x_server_t *s)
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
-    if (stream) hipStreamDestroy(stream);
 }
 
 void sequential_thread(moonlab_mi355x_server_t *s)
@@ -270,6 +268,12 @@ moonlab_mi355x_server_t *moonlab_mi355x_server_create(const moonlab_mi355x_serve
         s->threads.emplace_back(sequential_thread, s);
         return s;
     }
+    // The engines' streams and the large workers' streams are created here,
+    // one after another: HIP hands streams to its hardware queues in turn
+    // (4 queues per process by default, GPU_MAX_HW_QUEUES), so with the
+    // default two engines and two workers each stream gets a queue of its own
+    // and a batch launch never waits behind a large request's passes in one
+    // queue, whatever streams the process made before.
     for (int i = 0; i < s->cfg.batch_engines; i++) {
         moonlab_mi355x_batch_t *b = moonlab_mi355x_batch_create();
         if (!b) {
@@ -278,8 +282,16 @@ moonlab_mi355x_server_t *moonlab_mi355x_server_create(const moonlab_mi355x_serve
         }
         s->engines.push_back(b);
     }
+    for (int i = 0; i < s->cfg.large_workers; i++) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            moonlab_mi355x_server_free(s);
+            return nullptr;
+        }
+        s->large_streams.push_back(st);
+    }
     for (moonlab_mi355x_batch_t *b : s->engines) s->threads.emplace_back(batch_thread, s, b);
-    for (int i = 0; i < s->cfg.large_workers; i++) s->threads.emplace_back(large_thread, s);
+    for (hipStream_t st : s->large_streams) s->threads.emplace_back(large_thread, s, st);
     return s;
 }
 
@@ -294,6 +306,7 @@ void moonlab_mi355x_server_free(moonlab_mi355x_server_t *s)
     s->cv_done.notify_all();
     for (std::thread &t : s->threads) t.join();
     for (moonlab_mi355x_batch_t *b : s->engines) moonlab_mi355x_batch_free(b);
+    for (hipStream_t st : s->large_streams) hipStreamDestroy(st);
     for (auto &kv : s->reqs) delete kv.second;
     delete s;
 }

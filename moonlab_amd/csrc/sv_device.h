@@ -22,6 +22,17 @@ template <typename R> using cplx_t = typename Cplx<R>::type;
 __device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// c ? x : y per component: a select of whole structs can become a
+// dynamically indexed stack slot (scratch memory) instead of two v_cndmasks.
+template <typename T>
+__device__ __forceinline__ T csel(bool c, const T &x, const T &y)
+{
+    T r;
+    r.x = c ? x.x : y.x;
+    r.y = c ? x.y : y.y;
+    return r;
+}
+
 // Complex multiply (x.re, x.im) * (pr, pi) with a fixed FMA pattern.
 template <typename T, typename R>
 __device__ __forceinline__ T cmul(const T &x, R pr, R pi)

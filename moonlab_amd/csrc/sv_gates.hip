@@ -95,16 +95,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(cplx_t<R> *__restrict__ a, u
 // and each keeps its own half, so the results equal k_pair's bit for bit.
 // Launched with dim a multiple of the block size (n >= 8), so every wave is
 // full.
-// c ? x : y per component: a select of whole structs can become a
-// dynamically indexed stack slot (scratch memory) instead of two v_cndmasks.
-template <typename T>
-__device__ __forceinline__ T csel(bool c, const T &x, const T &y)
-{
-    T r;
-    r.x = c ? x.x : y.x;
-    r.y = c ? x.y : y.y;
-    return r;
-}
 
 template <typename R, typename Op, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_pair_lane(cplx_t<R> *__restrict__ a, uint64_t dim, int q,

@@ -25,10 +25,14 @@
 // D[4 v + l / 16][l % 16] for FP64.  The FP64 path loads A with its rows
 // permuted (A row i = real row 4 (i % 4) + i / 4), which puts both results in
 // the FP32 order, so one write-back serves both.
+// Complex128 5-qubit blocks clear of qubits 0..2 run k_dense_mfma_ring
+// instead (direct global->LDS loads into a 4-tile ring, results stored from
+// the accumulators; described at the kernel).
 #include "sv_device.h"
 #include "sv_internal.h"
 
 #include <atomic>
+#include <type_traits>
 #include <vector>
 
 namespace mlx {
@@ -132,9 +136,15 @@ constexpr int kThreads = 256;
 __device__ __forceinline__ int padded(int l) { return l + (l >> 5); }
 
 // Tile t's amplitude l lives at tile_base(t) | dlo[l % 64] | dhi[l / 64].
+// TB = G.tb is a compile-time constant of every kernel, so the tile set is
+// indexed with constants and read into scalar registers once; indexing it
+// with a loop variable made the compiler fetch each byte with a vector
+// load and wait for every outstanding load and store before using it.
+template <int TB>
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const DenseGeom &G)
 {
-    for (int b = 0; b < G.tb; b++) t = ins0(t, G.tq[b]);
+#pragma unroll
+    for (int b = 0; b < TB; b++) t = ins0(t, G.tq[b]);
     return t;
 }
 
@@ -199,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
     SlotT pre[PER];
     uint64_t t = blockIdx.x;
     if (t < ntiles) {
-        const uint64_t base = tile_base(t, G) | lo;
+        const uint64_t base = tile_base<K + 6 + GB>(t, G) | lo;
 #pragma unroll
         for (int i = 0; i < PER; i++) pre[i] = ld_slot<NT>((const SlotT *)&a[base | dhi[hi0 + HSTEP * i]]);
     }
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
         lds_barrier();
         const uint64_t nt = t + gridDim.x;
         if (nt < ntiles) {
-            const uint64_t nbase = tile_base(nt, G) | lo;
+            const uint64_t nbase = tile_base<K + 6 + GB>(nt, G) | lo;
 #pragma unroll
             for (int i = 0; i < PER; i++) pre[i] = ld_slot<NT>((const SlotT *)&a[nbase | dhi[hi0 + HSTEP * i]]);
         }
@@ -240,12 +250,221 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
             }
         }
         lds_barrier();
-        const uint64_t base = tile_base(t, G) | lo;
+        const uint64_t base = tile_base<K + 6 + GB>(t, G) | lo;
 #pragma unroll
         for (int i = 0; i < PER; i++)
             st_slot<NT>((SlotT *)&a[base | dhi[hi0 + HSTEP * i]], get_slot(lds, padded(A * (tid + kThreads * i))));
         lds_barrier();   // every wave has read its write-back values before the next tile lands
     }
+}
+
+// Ring variant (complex128, k = 5): tiles stream into a ring of kRing 32 KiB
+// LDS buffers with direct global->LDS loads (global_load_lds_dwordx4),
+// kRing - 1 tiles ahead of the contraction, one 256-thread workgroup per CU,
+// so 96 KiB per CU are in flight all the time and no VGPRs hold tiles in
+// flight.  The results go from the accumulators straight to HBM (no LDS
+// write-back, one barrier less per tile).  A wave's load instruction fills
+// 64 consecutive LDS slots, so LDS slot p holds tile slot swz(p), an XOR
+// within 32-amplitude rows that keeps the strided operand reads on distinct
+// banks as the padding above does; each half-wave's slots are still one
+// contiguous 512-byte run in HBM.
+constexpr int kRing = 4;
+
+template <typename R>
+__device__ __forceinline__ int ring_swz(int p)
+{
+    constexpr int SH = Slot<R>::A == 1 ? 5 : 4;   // 32 amplitudes per row
+    return p ^ ((p >> SH) & 15);
+}
+// complex index in a ring buffer of tile amplitude l
+template <typename R>
+__device__ __forceinline__ int ring_pos(int l)
+{
+    if constexpr (Slot<R>::A == 1) return ring_swz<R>(l);
+    else return 2 * ring_swz<R>(l >> 1) + (l & 1);
+}
+
+// s_waitcnt on vmcnt only (gfx9 encoding: vmcnt bits 3:0 and 15:14)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Workgroup barrier for the ring loop: LDS operations of this wave complete
+// (lgkmcnt), then s_barrier.  The fence-based lds_barrier() would also wait
+// for every direct global->LDS load in flight (they count as LDS writes),
+// draining the ring at every tile; the loop waits for its own tile's loads
+// with wait_vmcnt before this barrier instead.
+__device__ __forceinline__ void ring_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// The persistent loop of k_dense_mfma_ring.  Every LDS array it touches is a
+// separate __restrict__ parameter: the alias scopes this gives the accesses
+// let the compiler see that reading one ring buffer (or the index tables)
+// does not depend on the direct loads still filling the other buffers, so it
+// waits only where the loop asks it to.
+template <typename R, int K, int GB, bool NT>
+__device__ __forceinline__ void ring_loop(cplx_t<R> *__restrict__ a, uint64_t ntiles, const DenseGeom &G,
+                                          typename Slot<R>::type *__restrict__ ring0,
+                                          typename Slot<R>::type *__restrict__ ring1,
+                                          typename Slot<R>::type *__restrict__ ring2,
+                                          typename Slot<R>::type *__restrict__ ring3,
+                                          const uint16_t *__restrict__ offq, const uint16_t *__restrict__ offg_lo,
+                                          const uint16_t *__restrict__ offg_hi, const uint64_t *__restrict__ dlo,
+                                          const uint64_t *__restrict__ dhi,
+                                          const R (&Am)[(2 << K) / 16][(2 << K) / 4], const uint64_t (&off)[8],
+                                          const uint64_t (&gq)[(2 << K) / 16][2])
+{
+    constexpr int GM = 1 << GB;
+    constexpr int NRT = (2 << K) / 16;
+    constexpr int NS = (2 << K) / 4;
+    constexpr int PER = 8;
+    constexpr int ST = 2 * NRT * GM;                                    // stores per thread per tile
+    constexpr int AHEAD = 3 * PER + 3 * ST < 63 ? 3 * PER + 3 * ST : 63;   // memory ops issued after a tile's loads
+    using AccT = Acc<R>;
+    using SlotT = typename Slot<R>::type;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto buffer = [&](auto bc) -> SlotT * {
+        constexpr int b = decltype(bc)::value;
+        if constexpr (b == 0) return ring0;
+        else if constexpr (b == 1) return ring1;
+        else if constexpr (b == 2) return ring2;
+        else return ring3;
+    };
+    auto issue = [&](uint64_t t, SlotT *buf) {
+        const uint64_t base = tile_base<K + 6 + GB>(t, G);
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            __builtin_amdgcn_global_load_lds((const void *)&a[base | off[i]],
+                                             (__attribute__((address_space(3))) void *)&buf[64 * (wave + 4 * i)], 16,
+                                             0, 0);
+    };
+    const uint64_t step = gridDim.x;
+    uint64_t t = blockIdx.x;
+    if (t < ntiles) issue(t, ring0);
+    if (t + step < ntiles) issue(t + step, ring1);
+    if (t + 2 * step < ntiles) issue(t + 2 * step, ring2);
+
+    // tile t from buffer B; the load of tile t + 3 step goes to buffer B - 1,
+    // which every wave finished reading before the previous tile's last barrier
+    auto tile = [&](auto bc) -> bool {
+        constexpr int B = decltype(bc)::value;
+        if (t >= ntiles) return false;
+        const SlotT *buf = buffer(bc);
+        const uint64_t ahead = t + (kRing - 1) * step;
+        if (ahead < ntiles) {
+            issue(ahead, buffer(std::integral_constant<int, (B + kRing - 1) % kRing>{}));
+            wait_vmcnt<AHEAD>();   // tile t's loads are done (vmcnt retires in issue order)
+        } else {
+            wait_vmcnt<0>();
+        }
+        ring_barrier();
+        const uint64_t base = tile_base<K + 6 + GB>(t, G);
+        const R *ldsr = (const R *)buf;
+#pragma unroll 1
+        for (int j = 0; j < GM; j++) {
+            const int g = 64 * j + 16 * wave + (lane & 15);
+            const int go = offg_lo[g & 63] | offg_hi[g >> 6];
+            R Bv[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int kk = 4 * s + (lane >> 4);
+                Bv[s] = ldsr[2 * ring_pos<R>(offq[kk >> 1] | go) + (kk & 1)];
+            }
+            const uint64_t gg = base | dlo[go & 63] | dhi[go >> 6];
+            typename AccT::type acc[NRT];
+#pragma unroll
+            for (int rt = 0; rt < NRT; rt++) {
+                acc[rt] = typename AccT::type{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < NS; s++) acc[rt] = AccT::mma(Am[rt][s], Bv[s], acc[rt]);
+            }
+            // results straight from the accumulators to HBM: rows r0, r0 + 1 of group g
+#pragma unroll
+            for (int rt = 0; rt < NRT; rt++) {
+                st<NT>(&a[gg | gq[rt][0]], cplx_t<R>{acc[rt][0], acc[rt][1]});
+                st<NT>(&a[gg | gq[rt][1]], cplx_t<R>{acc[rt][2], acc[rt][3]});
+            }
+        }
+        ring_barrier();   // every wave has read this buffer before a load refills it
+        t += step;
+        return true;
+    };
+    while (tile(std::integral_constant<int, 0>{}) && tile(std::integral_constant<int, 1>{}) &&
+           tile(std::integral_constant<int, 2>{}) && tile(std::integral_constant<int, 3>{})) {
+    }
+}
+
+template <typename R, int K, int GB, bool NT>
+__global__ __launch_bounds__(kThreads, 1) void k_dense_mfma_ring(cplx_t<R> *__restrict__ a, uint64_t ntiles,
+                                                                 DenseGeom G, const cplx_t<R> *__restrict__ mat)
+{
+    constexpr int D = 1 << K;
+    constexpr int GM = 1 << GB;
+    constexpr int NRT = (2 * D) / 16;
+    constexpr int NS = (2 * D) / 4;
+    constexpr int T = 64 * GM * D;
+    constexpr int A = Slot<R>::A;
+    constexpr int NSLOT = T / A;             // 16-byte slots per tile
+    constexpr int PER = NSLOT / kThreads;    // slots each thread loads and stores per tile
+    static_assert(NSLOT == 2048 && PER == 8, "32 KiB tiles");
+    using AccT = Acc<R>;
+    using SlotT = typename Slot<R>::type;
+    // one variable per ring buffer, each addressed with a constant: the
+    // compiler then sees that reading one buffer does not depend on the
+    // direct loads still filling the others, and waits only on the counter
+    __shared__ SlotT ring0[NSLOT], ring1[NSLOT], ring2[NSLOT], ring3[NSLOT];
+    static_assert(kRing == 4, "four ring buffers");
+    __shared__ uint64_t dlo[64], dhi[64];
+    __shared__ uint16_t offq[32], offg_lo[64], offg_hi[8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    if (tid < 128) {
+        const int j = tid & 63, hi = tid >> 6;
+        uint64_t o = 0;
+        for (int b = 0; b < 6; b++) {
+            const int qi = hi * 6 + b;
+            if (qi < G.tb && ((j >> b) & 1)) o |= 1ull << G.tq[qi];
+        }
+        if (hi) dhi[j] = o;
+        else dlo[j] = o;
+    } else if (tid < 160) {
+        offq[tid - 128] = G.offq[tid - 128];
+    } else if (tid < 224) {
+        offg_lo[tid - 160] = G.offg_lo[tid - 160];
+    } else if (tid < 232) {
+        offg_hi[tid - 224] = G.offg_hi[tid - 224];
+    }
+    R Am[NRT][NS];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int row = 16 * rt + AccT::a_row(lane & 15), col = 4 * s + (lane >> 4);
+            const cplx_t<R> m = mat[(row >> 1) * D + (col >> 1)];
+            const int ra = row & 1, cb = col & 1;
+            Am[rt][s] = ra == cb ? m.x : (ra == 0 ? -m.y : m.y);
+        }
+    __syncthreads();
+
+    // load / store i of this thread: LDS slot 64 (wave + 4 i) + lane, tile slot swz of it
+    uint64_t off[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int l = A * ring_swz<R>(64 * (wave + 4 * i) + lane);
+        off[i] = dlo[l & 63] | dhi[l >> 6];
+    }
+    // global offsets of the result rows this lane writes: r0 = 8 rt + 2 (lane / 16) and r0 + 1
+    uint64_t gq[NRT][2];
+#pragma unroll
+    for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int lq = offq[8 * rt + 2 * (lane >> 4) + h];
+            gq[rt][h] = dlo[lq & 63] | dhi[lq >> 6];
+        }
+    ring_loop<R, K, GB, NT>(a, ntiles, G, ring0, ring1, ring2, ring3, offq, offg_lo, offg_hi, dlo, dhi, Am, off, gq);
 }
 
 // Tile set = Q plus the 6 + gb lowest other qubits; local indices follow the
@@ -330,10 +549,32 @@ int resident_blocks()
     return blocks;
 }
 
+// The ring kernel for complex128 5-qubit blocks whose qubits are all >= 3:
+// the three lowest qubits are then group bits, so the accumulator stores of
+// 8 neighbouring groups fill whole 128-byte lines.  Measured on 30-qubit
+// states (5 consecutive qubits at every position): 5.4-5.55 TB/s where the
+// staged kernel gives 4.3-5.1 TB/s for most positions (it reaches 5.6-6.1
+// for blocks starting at qubit 7..12, where the ring gives 5.5); with a block
+// on qubits 0..2 the ring's stores are partial lines (4.1-4.6 TB/s, the
+// staged kernel 5.0).  Complex64 stays on the staged kernel at every
+// position (ring 4.5-4.7 TB/s, staged 4.2-5.3).
 template <typename R, int K, int GB>
-hipError_t run_mfma(const SvDev &s, const DenseGeom &G, const void *d_mat, bool nt)
+hipError_t run_mfma(const SvDev &s, const DenseGeom &G, const void *d_mat, bool nt, bool low_groups)
 {
     const uint64_t ntiles = (1ull << s.n) >> G.tb;
+    if constexpr (sizeof(R) == 8 && K == 5 && GB == 0) {
+        if (low_groups) {
+            const int cus = device_cus();
+            const unsigned grid = (unsigned)(ntiles < (uint64_t)cus ? ntiles : (uint64_t)cus);
+            if (nt)
+                k_dense_mfma_ring<R, K, GB, true><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
+                                                                                   (const cplx_t<R> *)d_mat);
+            else
+                k_dense_mfma_ring<R, K, GB, false><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
+                                                                                    (const cplx_t<R> *)d_mat);
+            return hipGetLastError();
+        }
+    }
     const int mb = nt ? resident_blocks<R, K, GB, true>() : resident_blocks<R, K, GB, false>();
     const unsigned grid = (unsigned)(ntiles < (uint64_t)mb ? ntiles : (uint64_t)mb);
     if (nt)
@@ -353,9 +594,11 @@ template <typename R, int K>
 hipError_t dispatch_k(const SvDev &s, const int *qubits, const void *d_mat, bool nt)
 {
     constexpr int W = wide_gb(sizeof(R) == 4, K);
+    uint32_t low = 0;
+    for (int j = 0; j < K; j++) low |= qubits[j] < 3 ? 1u : 0u;
     DenseGeom G;
-    if (make_geom(s.n, K, W, qubits, G)) return run_mfma<R, K, W>(s, G, d_mat, nt);
-    if (make_geom(s.n, K, 0, qubits, G)) return run_mfma<R, K, 0>(s, G, d_mat, nt);
+    if (make_geom(s.n, K, W, qubits, G)) return run_mfma<R, K, W>(s, G, d_mat, nt, low == 0);
+    if (make_geom(s.n, K, 0, qubits, G)) return run_mfma<R, K, 0>(s, G, d_mat, nt, low == 0);
     return hipErrorInvalidValue;
 }
 

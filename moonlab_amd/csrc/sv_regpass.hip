@@ -56,29 +56,51 @@ __device__ __forceinline__ void relayout(cplx_t<R> *t, cplx_t<R> (&v)[1 << RB], 
     for (int j = 0; j < NV; j++) v[j] = *(const cplx_t<R> *)(tb + (s1 ^ to[j]));
 }
 
+// bit of a global qubit code (-1 - q) of the tile's base index; 0 for none
+__device__ __forceinline__ int global_bit(int32_t code, uint64_t base)
+{
+    return (code < 0 && code != INT32_MIN) ? (int)((base >> (-1 - code)) & 1ull) : 0;
+}
+
+// The ops of a register phase.  Each op is written from one register set into
+// another (every value of `out` is written: computed, or copied from `in`),
+// and the op loop alternates the two sets: each arm's results land where the
+// next op reads them, so the merge after the arm dispatch needs no register
+// copies; only the values an op leaves alone are copied.  (Updating one set
+// in place made the compiler keep a second copy of the set and copy all of
+// it back after every op.)
+//
 // 2x2 on register bit P for the register values j whose register-bit
-// controls hold (bit j of jm; MASKED false: all, with no branch between the
-// pairs, so their arithmetic interleaves).  FORM: 0 general (apply_op's
-// BOP_MAT1 arithmetic), 1 real entries (RY, H), 2 real diagonal and
-// imaginary off-diagonal (RX); forms 1 and 2 skip the zero products.
+// controls hold (bit j of jm; MASKED false: all).  FORM: 0 general (apply_op's
+// BOP_MAT1 arithmetic, m[0..7]), 1 real entries (RY, H: m[0..3] = u00, u01,
+// u10, u11), 2 real diagonal and imaginary off-diagonal (RX: m[0..3] = Re u00,
+// Im u01, Im u10, Re u11); forms 1 and 2 skip the zero products.
 template <typename R, int RB, int P, bool MASKED, int FORM>
-__device__ __forceinline__ void reg_mat1(cplx_t<R> (&v)[1 << RB], const BOp &op, uint32_t jm)
+__device__ __forceinline__ void pp_mat1(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], const double *m,
+                                        uint32_t jm)
 {
     constexpr int NV = 1 << RB;
-    const R m0 = (R)op.m[0], m1 = (R)op.m[1], m2 = (R)op.m[2], m3 = (R)op.m[3];
-    const R m4 = (R)op.m[4], m5 = (R)op.m[5], m6 = (R)op.m[6], m7 = (R)op.m[7];
+    // forms 1 and 2 keep the general form's names for their four entries:
+    // m0, m2, m4, m6 (real) and m0, m3, m5, m6 (RX)
+    const R m0 = (R)m[0];
+    const R m1 = FORM == 0 ? (R)m[1] : R(0);
+    const R m2 = FORM == 1 ? (R)m[1] : FORM == 0 ? (R)m[2] : R(0);
+    const R m3 = FORM == 2 ? (R)m[1] : FORM == 0 ? (R)m[3] : R(0);
+    const R m4 = FORM == 1 ? (R)m[2] : FORM == 0 ? (R)m[4] : R(0);
+    const R m5 = FORM == 2 ? (R)m[2] : FORM == 0 ? (R)m[5] : R(0);
+    const R m6 = FORM == 0 ? (R)m[6] : (R)m[3];
+    const R m7 = FORM == 0 ? (R)m[7] : R(0);
 #pragma unroll
     for (int j = 0; j < NV; j++) {
         if (j & (1 << P)) continue;
-        if (MASKED && !((jm >> j) & 1u)) continue;
-        const cplx_t<R> a = v[j], b = v[j | (1 << P)];
+        const cplx_t<R> a = in[j], b = in[j | (1 << P)];
         cplx_t<R> x, y;
         if (FORM == 1) {
             x.x = fmar(m0, a.x, m2 * b.x);
             x.y = fmar(m0, a.y, m2 * b.y);
             y.x = fmar(m4, a.x, m6 * b.x);
             y.y = fmar(m4, a.y, m6 * b.y);
-        } else if (FORM == 2) {   // u00 = m0, u01 = i m3, u10 = i m5, u11 = m6
+        } else if (FORM == 2) {
             x.x = fmar(m0, a.x, -(m3 * b.y));
             x.y = fmar(m0, a.y, m3 * b.x);
             y.x = fmar(m6, b.x, -(m5 * a.y));
@@ -89,159 +111,207 @@ __device__ __forceinline__ void reg_mat1(cplx_t<R> (&v)[1 << RB], const BOp &op,
             y.x = fmar(m4, a.x, -(m5 * a.y)) + fmar(m6, b.x, -(m7 * b.y));
             y.y = fmar(m4, a.y, m5 * a.x) + fmar(m6, b.y, m7 * b.x);
         }
-        v[j] = x;
-        v[j | (1 << P)] = y;
+        if (MASKED && !((jm >> j) & 1u)) {
+            out[j] = a;
+            out[j | (1 << P)] = b;
+        } else {
+            out[j] = x;
+            out[j | (1 << P)] = y;
+        }
     }
 }
 
-// controlled X on register bit P for the values selected by jm (any
-// controls)
+// controlled X on register bit P for the values selected by jm (any controls)
 template <typename R, int RB, int P>
-__device__ __forceinline__ void reg_xperm(cplx_t<R> (&v)[1 << RB], uint32_t jm)
+__device__ __forceinline__ void pp_xperm(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], uint32_t jm)
 {
     constexpr int NV = 1 << RB;
 #pragma unroll
     for (int j = 0; j < NV; j++) {
         if (j & (1 << P)) continue;
-        if (!((jm >> j) & 1u)) continue;
-        const cplx_t<R> a = v[j];
-        v[j] = v[j | (1 << P)];
-        v[j | (1 << P)] = a;
+        const bool sw = (jm >> j) & 1u;
+        out[j] = csel(sw, in[j | (1 << P)], in[j]);
+        out[j | (1 << P)] = csel(sw, in[j], in[j | (1 << P)]);
     }
 }
 
 // CNOT with target register bit P and control register bit C: a fixed
 // exchange of register values
 template <typename R, int RB, int P, int C>
-__device__ __forceinline__ void reg_cx(cplx_t<R> (&v)[1 << RB])
+__device__ __forceinline__ void pp_cx(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB])
 {
     constexpr int NV = 1 << RB;
 #pragma unroll
-    for (int j = 0; j < NV; j++) {
-        if ((j & (1 << P)) || !(j & (1 << C))) continue;
-        const cplx_t<R> a = v[j];
-        v[j] = v[j | (1 << P)];
-        v[j | (1 << P)] = a;
-    }
+    for (int j = 0; j < NV; j++) out[j] = in[(j & (1 << C)) ? (j ^ (1 << P)) : j];
 }
 
 // single-qubit diagonal on register bit P: value j *= d[bit P of j]
-// (ONLY1: d0 is exactly 1 and those values are left alone)
+// (ONLY1: d0 is exactly 1 and those values are copied)
 template <typename R, int RB, int P, bool ONLY1>
-__device__ __forceinline__ void reg_diag1(cplx_t<R> (&v)[1 << RB], const BOp &op)
+__device__ __forceinline__ void pp_diag1(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], const double *m)
 {
     constexpr int NV = 1 << RB;
-    const R d0r = (R)op.m[0], d0i = (R)op.m[1], d1r = (R)op.m[2], d1i = (R)op.m[3];
+    const R d0r = (R)m[0], d0i = (R)m[1], d1r = (R)m[2], d1i = (R)m[3];
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-        if (j & (1 << P)) v[j] = cmul(v[j], d1r, d1i);
-        else if (!ONLY1) v[j] = cmul(v[j], d0r, d0i);
+        if (j & (1 << P)) out[j] = cmul(in[j], d1r, d1i);
+        else out[j] = ONLY1 ? in[j] : cmul(in[j], d0r, d0i);
     }
 }
 
 // two-qubit diagonal diag(1, 1, 1, d3) on register bits P < Q (CZ, CPHASE)
 template <typename R, int RB, int P, int Q>
-__device__ __forceinline__ void reg_diag11(cplx_t<R> (&v)[1 << RB], const BOp &op)
+__device__ __forceinline__ void pp_diag11(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], const double *m)
 {
     constexpr int NV = 1 << RB;
-    const R d3r = (R)op.m[6], d3i = (R)op.m[7];
+    const R d3r = (R)m[0], d3i = (R)m[1];
 #pragma unroll
-    for (int j = 0; j < NV; j++)
-        if ((j & (1 << P)) && (j & (1 << Q))) v[j] = cmul(v[j], d3r, d3i);
+    for (int j = 0; j < NV; j++) out[j] = ((j & (1 << P)) && (j & (1 << Q))) ? cmul(in[j], d3r, d3i) : in[j];
 }
 
 // any diagonal op: value j *= d[k], k = kg | bits 2j..2j+1 of kmap (the
 // local qubits' bits of register value j); kg holds the global qubits' bits
 template <typename R, int RB>
-__device__ __forceinline__ void reg_diag(cplx_t<R> (&v)[1 << RB], const BOp &op, uint32_t jm, uint32_t kmap, int kg)
+__device__ __forceinline__ void pp_diag(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], const BOp &op,
+                                        uint32_t jm, uint32_t kmap, int kg)
 {
     constexpr int NV = 1 << RB;
     MLX_DIAG4(R, op);
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-        if (!((jm >> j) & 1u)) continue;
         const int k = kg | (int)((kmap >> (2 * j)) & 3u);
-        v[j] = cmul(v[j], pick4(k, dr0, dr1, dr2, dr3), pick4(k, di0, di1, di2, di3));
+        const cplx_t<R> d = cmul(in[j], pick4(k, dr0, dr1, dr2, dr3), pick4(k, di0, di1, di2, di3));
+        out[j] = csel(((jm >> j) & 1u) != 0, d, in[j]);
     }
 }
 
-// bit of a global qubit code (-1 - q) of the tile's base index; 0 for none
-__device__ __forceinline__ int global_bit(int32_t code, uint64_t base)
+template <typename R, int RB>
+__device__ __forceinline__ void pp_copy(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB])
 {
-    return (code < 0 && code != INT32_MIN) ? (int)((base >> (-1 - code)) & 1ull) : 0;
+#pragma unroll
+    for (int j = 0; j < (1 << RB); j++) out[j] = in[j];
 }
 
-// One op of a register phase, dispatched on its arm (BOp::pad, chosen by the
-// host; the numbering is RegArm in batch_device.h).  The planner places every
-// tile qubit an op touches in the phase's register bits, so the op is the
-// same for every thread of the tile.
-template <typename R, int RB>
-__device__ __forceinline__ void reg_op(cplx_t<R> (&v)[1 << RB], const BOp &op, uint64_t base)
+// The fields of an op that every common arm reads (the arm, jm, cg and
+// m[0..3]: one 64-byte scalar load), fetched one op ahead of their use.
+struct OpHead {
+    int32_t  arm;
+    uint32_t jm;
+    uint64_t cg;
+    double   m[4];
+};
+
+__device__ __forceinline__ OpHead op_head(const BOp *o)
 {
-    constexpr int NV = 1 << RB;
+    OpHead h;
+    h.arm = o->pad;
+    h.jm = o->pad2;
+    h.cg = o->cg;
+#pragma unroll
+    for (int i = 0; i < 4; i++) h.m[i] = o->m[i];
+    return h;
+}
+
+// One op, dispatched on its arm (BOp::pad, chosen by the host; the numbering
+// is RegArm in batch_device.h).  The planner places every tile qubit an op
+// touches in the phase's register bits, so the op is the same for every
+// thread of the tile.  The general arms read m[4..7] (and ARM_DIAG its other
+// fields) from the op itself.
+template <typename R, int RB>
+__device__ __forceinline__ void pp_op(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], const OpHead &h,
+                                      const BOp *op, uint64_t base)
+{
     static_assert(RB == 3 || RB == 4, "register positions 0..RB-1");
-    if ((base & op.cg) != op.cg) return;   // a global control is 0 for this whole tile
-    const uint32_t jm = op.pad2;
-    switch (op.pad) {
-    case ARM_MAT1_MASKED + 0: reg_mat1<R, RB, 0, true, 0>(v, op, jm); break;
-    case ARM_MAT1_MASKED + 1: reg_mat1<R, RB, 1, true, 0>(v, op, jm); break;
-    case ARM_MAT1_MASKED + 2: reg_mat1<R, RB, 2, true, 0>(v, op, jm); break;
-    case ARM_MAT1_MASKED + 3: if constexpr (RB > 3) reg_mat1<R, RB, 3, true, 0>(v, op, jm); break;
-    case ARM_MAT1 + 0: reg_mat1<R, RB, 0, false, 0>(v, op, jm); break;
-    case ARM_MAT1 + 1: reg_mat1<R, RB, 1, false, 0>(v, op, jm); break;
-    case ARM_MAT1 + 2: reg_mat1<R, RB, 2, false, 0>(v, op, jm); break;
-    case ARM_MAT1 + 3: if constexpr (RB > 3) reg_mat1<R, RB, 3, false, 0>(v, op, jm); break;
-    case ARM_MAT1_REAL + 0: reg_mat1<R, RB, 0, false, 1>(v, op, jm); break;
-    case ARM_MAT1_REAL + 1: reg_mat1<R, RB, 1, false, 1>(v, op, jm); break;
-    case ARM_MAT1_REAL + 2: reg_mat1<R, RB, 2, false, 1>(v, op, jm); break;
-    case ARM_MAT1_REAL + 3: if constexpr (RB > 3) reg_mat1<R, RB, 3, false, 1>(v, op, jm); break;
-    case ARM_MAT1_RX + 0: reg_mat1<R, RB, 0, false, 2>(v, op, jm); break;
-    case ARM_MAT1_RX + 1: reg_mat1<R, RB, 1, false, 2>(v, op, jm); break;
-    case ARM_MAT1_RX + 2: reg_mat1<R, RB, 2, false, 2>(v, op, jm); break;
-    case ARM_MAT1_RX + 3: if constexpr (RB > 3) reg_mat1<R, RB, 3, false, 2>(v, op, jm); break;
-    case ARM_XPERM + 0: reg_xperm<R, RB, 0>(v, jm); break;
-    case ARM_XPERM + 1: reg_xperm<R, RB, 1>(v, jm); break;
-    case ARM_XPERM + 2: reg_xperm<R, RB, 2>(v, jm); break;
-    case ARM_XPERM + 3: if constexpr (RB > 3) reg_xperm<R, RB, 3>(v, jm); break;
-    case ARM_CX + 4 * 0 + 1: reg_cx<R, RB, 0, 1>(v); break;
-    case ARM_CX + 4 * 0 + 2: reg_cx<R, RB, 0, 2>(v); break;
-    case ARM_CX + 4 * 0 + 3: if constexpr (RB > 3) reg_cx<R, RB, 0, 3>(v); break;
-    case ARM_CX + 4 * 1 + 0: reg_cx<R, RB, 1, 0>(v); break;
-    case ARM_CX + 4 * 1 + 2: reg_cx<R, RB, 1, 2>(v); break;
-    case ARM_CX + 4 * 1 + 3: if constexpr (RB > 3) reg_cx<R, RB, 1, 3>(v); break;
-    case ARM_CX + 4 * 2 + 0: reg_cx<R, RB, 2, 0>(v); break;
-    case ARM_CX + 4 * 2 + 1: reg_cx<R, RB, 2, 1>(v); break;
-    case ARM_CX + 4 * 2 + 3: if constexpr (RB > 3) reg_cx<R, RB, 2, 3>(v); break;
-    case ARM_CX + 4 * 3 + 0: if constexpr (RB > 3) reg_cx<R, RB, 3, 0>(v); break;
-    case ARM_CX + 4 * 3 + 1: if constexpr (RB > 3) reg_cx<R, RB, 3, 1>(v); break;
-    case ARM_CX + 4 * 3 + 2: if constexpr (RB > 3) reg_cx<R, RB, 3, 2>(v); break;
-    case ARM_DIAG1 + 0: reg_diag1<R, RB, 0, false>(v, op); break;
-    case ARM_DIAG1 + 1: reg_diag1<R, RB, 1, false>(v, op); break;
-    case ARM_DIAG1 + 2: reg_diag1<R, RB, 2, false>(v, op); break;
-    case ARM_DIAG1 + 3: if constexpr (RB > 3) reg_diag1<R, RB, 3, false>(v, op); break;
-    case ARM_DIAG1_ONLY1 + 0: reg_diag1<R, RB, 0, true>(v, op); break;
-    case ARM_DIAG1_ONLY1 + 1: reg_diag1<R, RB, 1, true>(v, op); break;
-    case ARM_DIAG1_ONLY1 + 2: reg_diag1<R, RB, 2, true>(v, op); break;
-    case ARM_DIAG1_ONLY1 + 3: if constexpr (RB > 3) reg_diag1<R, RB, 3, true>(v, op); break;
-    case ARM_DIAG11 + 4 * 0 + 1: reg_diag11<R, RB, 0, 1>(v, op); break;
-    case ARM_DIAG11 + 4 * 0 + 2: reg_diag11<R, RB, 0, 2>(v, op); break;
-    case ARM_DIAG11 + 4 * 0 + 3: if constexpr (RB > 3) reg_diag11<R, RB, 0, 3>(v, op); break;
-    case ARM_DIAG11 + 4 * 1 + 2: reg_diag11<R, RB, 1, 2>(v, op); break;
-    case ARM_DIAG11 + 4 * 1 + 3: if constexpr (RB > 3) reg_diag11<R, RB, 1, 3>(v, op); break;
-    case ARM_DIAG11 + 4 * 2 + 3: if constexpr (RB > 3) reg_diag11<R, RB, 2, 3>(v, op); break;
+    // arm -1: a global control is 0 for this whole tile
+    const int arm = (base & h.cg) == h.cg ? h.arm : -1;
+    const uint32_t jm = h.jm;
+    const double *m = h.m;
+    if (arm < 0) {
+        pp_copy<R, RB>(in, out);
+        return;
+    }
+    double m8[8];   // the general arms' eight entries
+    auto full = [&]() -> const double * {
+#pragma unroll
+        for (int i = 0; i < 4; i++) m8[i] = h.m[i];
+#pragma unroll
+        for (int i = 4; i < 8; i++) m8[i] = op->m[i];
+        return m8;
+    };
+#define MLX_PP4(ARM, CALL)                                                         \
+    case ARM + 0: CALL(0); break;                                                  \
+    case ARM + 1: CALL(1); break;                                                  \
+    case ARM + 2: CALL(2); break;                                                  \
+    case ARM + 3:                                                                  \
+        if constexpr (RB > 3) CALL(3);                                             \
+        else pp_copy<R, RB>(in, out);                                              \
+        break;
+#define MLX_PP_MASKED(P) pp_mat1<R, RB, P, true, 0>(in, out, full(), jm)
+#define MLX_PP_MAT1(P) pp_mat1<R, RB, P, false, 0>(in, out, full(), jm)
+#define MLX_PP_REAL(P) pp_mat1<R, RB, P, false, 1>(in, out, m, jm)
+#define MLX_PP_RX(P) pp_mat1<R, RB, P, false, 2>(in, out, m, jm)
+#define MLX_PP_XPERM(P) pp_xperm<R, RB, P>(in, out, jm)
+#define MLX_PP_DIAG1(P) pp_diag1<R, RB, P, false>(in, out, m)
+#define MLX_PP_ONLY1(P) pp_diag1<R, RB, P, true>(in, out, m)
+#define MLX_PP_PAIR(ARM, P, Q, CALL)                                               \
+    case ARM + 4 * P + Q:                                                          \
+        if constexpr (RB > 3 || (P < 3 && Q < 3)) CALL<R, RB, P, Q>(in, out);      \
+        else pp_copy<R, RB>(in, out);                                              \
+        break;
+#define MLX_PP_PAIR_OP(ARM, P, Q, CALL)                                            \
+    case ARM + 4 * P + Q:                                                          \
+        if constexpr (RB > 3 || (P < 3 && Q < 3)) CALL<R, RB, P, Q>(in, out, m);   \
+        else pp_copy<R, RB>(in, out);                                              \
+        break;
+    switch (arm) {
+        MLX_PP4(ARM_MAT1_MASKED, MLX_PP_MASKED)
+        MLX_PP4(ARM_MAT1, MLX_PP_MAT1)
+        MLX_PP4(ARM_MAT1_REAL, MLX_PP_REAL)
+        MLX_PP4(ARM_MAT1_RX, MLX_PP_RX)
+        MLX_PP4(ARM_XPERM, MLX_PP_XPERM)
+        MLX_PP_PAIR(ARM_CX, 0, 1, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 0, 2, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 0, 3, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 1, 0, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 1, 2, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 1, 3, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 2, 0, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 2, 1, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 2, 3, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 3, 0, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 3, 1, pp_cx)
+        MLX_PP_PAIR(ARM_CX, 3, 2, pp_cx)
+        MLX_PP4(ARM_DIAG1, MLX_PP_DIAG1)
+        MLX_PP4(ARM_DIAG1_ONLY1, MLX_PP_ONLY1)
+        MLX_PP_PAIR_OP(ARM_DIAG11, 0, 1, pp_diag11)
+        MLX_PP_PAIR_OP(ARM_DIAG11, 0, 2, pp_diag11)
+        MLX_PP_PAIR_OP(ARM_DIAG11, 0, 3, pp_diag11)
+        MLX_PP_PAIR_OP(ARM_DIAG11, 1, 2, pp_diag11)
+        MLX_PP_PAIR_OP(ARM_DIAG11, 1, 3, pp_diag11)
+        MLX_PP_PAIR_OP(ARM_DIAG11, 2, 3, pp_diag11)
     default: {   // ARM_DIAG
-        const int kg = global_bit(op.lq, base) | (global_bit(op.lq2, base) << 1);
-        reg_diag<R, RB>(v, op, jm, op.cl, __builtin_amdgcn_readfirstlane(kg));
+        const int kg = global_bit(op->lq, base) | (global_bit(op->lq2, base) << 1);
+        pp_diag<R, RB>(in, out, *op, jm, op->cl, __builtin_amdgcn_readfirstlane(kg));
         break;
     }
     }
+#undef MLX_PP4
+#undef MLX_PP_MASKED
+#undef MLX_PP_MAT1
+#undef MLX_PP_REAL
+#undef MLX_PP_RX
+#undef MLX_PP_XPERM
+#undef MLX_PP_DIAG1
+#undef MLX_PP_ONLY1
+#undef MLX_PP_PAIR
+#undef MLX_PP_PAIR_OP
 }
 
 // One pass.  A tile enters and leaves in the IO layout, whose register bits
 // are the top RB tile bits: the lanes of a wave then cover the low tile
 // bits, which are runs of contiguous amplitudes.
-template <typename R, bool NT, int TB, int RB>
-__global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restrict__ a, BPass P,
+template <typename R, bool NT, int TB, int RB, int WPE>
+__global__ __launch_bounds__(1 << (TB - RB), WPE) void k_reg_pass(cplx_t<R> *__restrict__ a, BPass P,
                                                             const BOp *__restrict__ ops, uint64_t ntiles)
 {
     constexpr int NV = 1 << RB, THREADS = 1 << (TB - RB), HSTEP = THREADS / 64, NHI = 1 << (TB - 6);
@@ -315,7 +385,35 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restri
                 cur_mask = mask;
             }
             const int cnt = h.lq;
-            for (int q = 1; q <= cnt; q++) reg_op<R, RB>(v, pops[k + q], base);
+            // ops alternate between the two register sets (v -> w -> v);
+            // each op's head is loaded while the op before it runs: its
+            // address depends on the previous head's arm (always >= 0), so
+            // the load issues only once that head has arrived, and the wait
+            // before an op never includes the load of the op after it
+            // (complex64 ops are short enough that the prefetch's extra
+            // registers cost more than the latency it hides: measured 10 %
+            // slower, so complex64 loads each head where it is used)
+            cplx_t<R> w[NV];
+            const BOp *po = pops + k;
+            if constexpr (sizeof(R) == 8) {
+                OpHead h0 = op_head(po + 1);
+                for (int q = 1; q <= cnt; q += 2) {
+                    const OpHead h1 = op_head(po + (q < cnt ? q + 1 : q) + (h0.arm >> 31));
+                    pp_op<R, RB>(v, w, h0, po + q, base);
+                    if (q < cnt) {
+                        h0 = op_head(po + (q + 1 < cnt ? q + 2 : q + 1) + (h1.arm >> 31));
+                        pp_op<R, RB>(w, v, h1, po + q + 1, base);
+                    } else {
+                        pp_copy<R, RB>(w, v);
+                    }
+                }
+            } else {
+                for (int q = 1; q <= cnt; q += 2) {
+                    pp_op<R, RB>(v, w, op_head(po + q), po + q, base);
+                    if (q < cnt) pp_op<R, RB>(w, v, op_head(po + q + 1), po + q + 1, base);
+                    else pp_copy<R, RB>(w, v);
+                }
+            }
             k += 1 + cnt;
         }
         if (cur_mask != io_mask) relayout<R, RB>(t, v, tloc, cur_off, io_off, io);
@@ -324,14 +422,14 @@ __global__ __launch_bounds__(1 << (TB - RB)) void k_reg_pass(cplx_t<R> *__restri
     }
 }
 
-template <typename R, bool NT, int TB, int RB>
-hipError_t run_reg_pass(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, hipStream_t st)
+template <typename R, bool NT, int TB, int RB, int WPE>
+hipError_t run_reg_pass_w(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, hipStream_t st)
 {
     constexpr int THREADS = 1 << (TB - RB);
     static std::atomic<int> per_cu_cached{0};
     int per_cu = per_cu_cached.load(std::memory_order_relaxed);
     if (!per_cu) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_reg_pass<R, NT, TB, RB>, THREADS, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_reg_pass<R, NT, TB, RB, WPE>, THREADS, 0) !=
                 hipSuccess ||
             per_cu < 1)
             per_cu = 1;
@@ -339,8 +437,17 @@ hipError_t run_reg_pass(void *amps, const BPass &P, const BOp *d_ops, uint64_t n
     }
     const uint64_t want = (uint64_t)cus * (uint64_t)per_cu;
     const unsigned grid = (unsigned)(ntiles < want ? ntiles : want);
-    k_reg_pass<R, NT, TB, RB><<<grid, THREADS, 0, st>>>((cplx_t<R> *)amps, P, d_ops, ntiles);
+    k_reg_pass<R, NT, TB, RB, WPE><<<grid, THREADS, 0, st>>>((cplx_t<R> *)amps, P, d_ops, ntiles);
     return hipGetLastError();
+}
+
+template <typename R, bool NT, int TB, int RB>
+hipError_t run_reg_pass(void *amps, const BPass &P, const BOp *d_ops, uint64_t ntiles, int cus, hipStream_t st)
+{
+    // complex64 tiles of 2^(TB - RB) >= 512 threads: two workgroups fit a
+    // CU's LDS, so 4 waves per SIMD, which needs <= 128 VGPRs
+    constexpr int WPE = (sizeof(R) == 4 && TB - RB >= 9) ? 4 : 1;
+    return run_reg_pass_w<R, NT, TB, RB, WPE>(amps, P, d_ops, ntiles, cus, st);
 }
 
 template <typename R, bool NT>

@@ -123,8 +123,8 @@ static void reg_tile(std::vector<cd> &tile, int TB, int RB, const mlx::BOp *ops,
                         if (masked && ((jm >> j) & 1) != ((jm >> (j | (1 << P))) & 1)) { printf("control mask differs across the target\n"); exit(1); }
                         if (masked && !((jm >> j) & 1)) continue;
                         cd &a = w[j], &b = w[j | (1 << P)];
-                        if (form == 1) { double m0 = op.m[0], m2 = op.m[2], m4 = op.m[4], m6 = op.m[6]; cd x = m0 * a + m2 * b, y = m4 * a + m6 * b; a = x; b = y; }
-                        else if (form == 2) { cd u01(0, op.m[3]), u10(0, op.m[5]); cd x = op.m[0] * a + u01 * b, y = u10 * a + op.m[6] * b; a = x; b = y; }
+                        if (form == 1) { double m0 = op.m[0], m2 = op.m[1], m4 = op.m[2], m6 = op.m[3]; cd x = m0 * a + m2 * b, y = m4 * a + m6 * b; a = x; b = y; }
+                        else if (form == 2) { cd u01(0, op.m[1]), u10(0, op.m[2]); cd x = op.m[0] * a + u01 * b, y = u10 * a + op.m[3] * b; a = x; b = y; }
                         else mat1(a, b, op);
                     }
                 } else if (arm < mlx::ARM_CX) {   // XPERM on jm
@@ -149,7 +149,7 @@ static void reg_tile(std::vector<cd> &tile, int TB, int RB, const mlx::BOp *ops,
                 } else if (arm < mlx::ARM_DIAG) {   // diag(1, 1, 1, d3) on P < Q
                     const int P = (arm - mlx::ARM_DIAG11) / 4, Q = (arm - mlx::ARM_DIAG11) % 4;
                     if (op.kind != mlx::BOP_DIAG || P >= Q || Q >= RB) { printf("DIAG11 arm mismatch\n"); exit(1); }
-                    for (int j = 0; j < NV; j++) if ((j & (1 << P)) && (j & (1 << Q))) w[j] *= cd(op.m[6], op.m[7]);
+                    for (int j = 0; j < NV; j++) if ((j & (1 << P)) && (j & (1 << Q))) w[j] *= cd(op.m[0], op.m[1]);
                 } else if (arm == mlx::ARM_DIAG) {
                     if (op.kind != mlx::BOP_DIAG) { printf("DIAG arm on a non-diagonal op\n"); exit(1); }
                     auto gbit = [&](int32_t c) { return (c < 0 && c != INT32_MIN) ? (int)((base >> (-1 - c)) & 1) : 0; };
