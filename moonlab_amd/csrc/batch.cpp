@@ -1048,6 +1048,10 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
             cus = 256;
         cached_cus.store(cus, std::memory_order_relaxed);
     }
+    // k_reg_pass fetches the head of the op after each op it runs (the next
+    // phase's header after a phase's last op): one spare op after the last
+    // keeps that fetch inside the buffer
+    if (reg) ops.push_back(BOp{});
     BOp *d_ops = nullptr;
     hipError_t e = hipMallocAsync((void **)&d_ops, ops.size() * sizeof(BOp), st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st);

@@ -148,6 +148,24 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const DenseGeom &G)
     return t;
 }
 
+// The tiles a persistent workgroup takes: t, t + step, ... below end.  The
+// dispatcher hands consecutive workgroups to the 8 XCDs in turn, so when the
+// grid and the tile count split evenly, workgroup b (on XCD b % 8) walks the
+// b % 8-th eighth of the tiles and each XCD streams one contiguous range;
+// otherwise workgroup b takes b, b + grid, ...
+struct TileWalk {
+    uint64_t t, step, end;
+};
+__device__ __forceinline__ TileWalk tile_walk(uint64_t ntiles)
+{
+    const uint64_t g = gridDim.x, b = blockIdx.x;
+    if ((g & 7) == 0 && (ntiles & 7) == 0 && ntiles >= g) {
+        const uint64_t per = ntiles >> 3;
+        return TileWalk{(b & 7) * per + (b >> 3), g >> 3, ((b & 7) + 1) * per};
+    }
+    return TileWalk{b, g, ntiles};
+}
+
 template <typename R, int K, int GB, bool NT>
 __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__ a, uint64_t ntiles,
                                                          DenseGeom G, const cplx_t<R> *__restrict__ mat)
@@ -204,21 +222,22 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
     const int hi0 = tid / (64 / A);
     const R *ldsr = (const R *)lds;
 
-    // software pipeline: tile t + gridDim.x is in flight from global memory
-    // while tile t is contracted and written back
+    // software pipeline: the walk's next tile is in flight from global
+    // memory while tile t is contracted and written back
     SlotT pre[PER];
-    uint64_t t = blockIdx.x;
-    if (t < ntiles) {
+    const TileWalk walk = tile_walk(ntiles);
+    uint64_t t = walk.t;
+    if (t < walk.end) {
         const uint64_t base = tile_base<K + 6 + GB>(t, G) | lo;
 #pragma unroll
         for (int i = 0; i < PER; i++) pre[i] = ld_slot<NT>((const SlotT *)&a[base | dhi[hi0 + HSTEP * i]]);
     }
-    for (; t < ntiles; t += gridDim.x) {
+    for (; t < walk.end; t += walk.step) {
 #pragma unroll
         for (int i = 0; i < PER; i++) put_slot(lds, padded(A * (tid + kThreads * i)), pre[i]);
         lds_barrier();
-        const uint64_t nt = t + gridDim.x;
-        if (nt < ntiles) {
+        const uint64_t nt = t + walk.step;
+        if (nt < walk.end) {
             const uint64_t nbase = tile_base<K + 6 + GB>(nt, G) | lo;
 #pragma unroll
             for (int i = 0; i < PER; i++) pre[i] = ld_slot<NT>((const SlotT *)&a[nbase | dhi[hi0 + HSTEP * i]]);
@@ -340,20 +359,21 @@ __device__ __forceinline__ void ring_loop(cplx_t<R> *__restrict__ a, uint64_t nt
                                              (__attribute__((address_space(3))) void *)&buf[64 * (wave + 4 * i)], 16,
                                              0, 0);
     };
-    const uint64_t step = gridDim.x;
-    uint64_t t = blockIdx.x;
-    if (t < ntiles) issue(t, ring0);
-    if (t + step < ntiles) issue(t + step, ring1);
-    if (t + 2 * step < ntiles) issue(t + 2 * step, ring2);
+    const TileWalk walk = tile_walk(ntiles);
+    const uint64_t step = walk.step, end = walk.end;
+    uint64_t t = walk.t;
+    if (t < end) issue(t, ring0);
+    if (t + step < end) issue(t + step, ring1);
+    if (t + 2 * step < end) issue(t + 2 * step, ring2);
 
     // tile t from buffer B; the load of tile t + 3 step goes to buffer B - 1,
     // which every wave finished reading before the previous tile's last barrier
     auto tile = [&](auto bc) -> bool {
         constexpr int B = decltype(bc)::value;
-        if (t >= ntiles) return false;
+        if (t >= end) return false;
         const SlotT *buf = buffer(bc);
         const uint64_t ahead = t + (kRing - 1) * step;
-        if (ahead < ntiles) {
+        if (ahead < end) {
             issue(ahead, buffer(std::integral_constant<int, (B + kRing - 1) % kRing>{}));
             wait_vmcnt<AHEAD>();   // tile t's loads are done (vmcnt retires in issue order)
         } else {

@@ -201,6 +201,13 @@ struct OpHead {
     double   m[4];
 };
 
+// p itself, with an address that depends on `arm` (always >= 0): a fetch
+// from it issues only once `arm` has arrived
+__device__ __forceinline__ const BOp *after(const BOp *p, int32_t arm)
+{
+    return (const BOp *)((const char *)p + ((arm >> 31) & 64));
+}
+
 __device__ __forceinline__ OpHead op_head(const BOp *o)
 {
     OpHead h;
@@ -387,8 +394,8 @@ __global__ __launch_bounds__(1 << (TB - RB), WPE) void k_reg_pass(cplx_t<R> *__r
             const int cnt = h.lq;
             // ops alternate between the two register sets (v -> w -> v);
             // each op's head is loaded while the op before it runs: its
-            // address depends on the previous head's arm (always >= 0), so
-            // the load issues only once that head has arrived, and the wait
+            // address depends on the previous head's arm (after()), so the
+            // load issues only once that head has arrived, and the wait
             // before an op never includes the load of the op after it
             // (complex64 ops are short enough that the prefetch's extra
             // registers cost more than the latency it hides: measured 10 %
@@ -396,13 +403,17 @@ __global__ __launch_bounds__(1 << (TB - RB), WPE) void k_reg_pass(cplx_t<R> *__r
             cplx_t<R> w[NV];
             const BOp *po = pops + k;
             if constexpr (sizeof(R) == 8) {
-                OpHead h0 = op_head(po + 1);
-                for (int q = 1; q <= cnt; q += 2) {
-                    const OpHead h1 = op_head(po + (q < cnt ? q + 1 : q) + (h0.arm >> 31));
-                    pp_op<R, RB>(v, w, h0, po + q, base);
+                // the op after a phase's last op is the next phase's header
+                // (or the spare op the host puts after the last op): fetched,
+                // never run
+                const BOp *p = po + 1;
+                OpHead h0 = op_head(p);
+                for (int q = 1; q <= cnt; q += 2, p += 2) {
+                    const OpHead h1 = op_head(after(p + 1, h0.arm));
+                    pp_op<R, RB>(v, w, h0, p, base);
                     if (q < cnt) {
-                        h0 = op_head(po + (q + 1 < cnt ? q + 2 : q + 1) + (h1.arm >> 31));
-                        pp_op<R, RB>(w, v, h1, po + q + 1, base);
+                        h0 = op_head(after(p + 2, h1.arm));
+                        pp_op<R, RB>(w, v, h1, p + 1, base);
                     } else {
                         pp_copy<R, RB>(w, v);
                     }
