@@ -134,6 +134,24 @@ __device__ __forceinline__ R pick4(int k, R a, R b, R c, R d)
     const R dr2 = wave_uniform((R)(op).m[4]), di2 = wave_uniform((R)(op).m[5]);                           \
     const R dr3 = wave_uniform((R)(op).m[6]), di3 = wave_uniform((R)(op).m[7])
 
+// The tiles a persistent workgroup takes: t, t + step, ... below end.  The
+// dispatcher hands consecutive workgroups to the 8 XCDs in turn, so when the
+// grid and the tile count split evenly, workgroup b (on XCD b % 8) walks the
+// b % 8-th eighth of the tiles and each XCD streams one contiguous range;
+// otherwise workgroup b takes b, b + grid, ...
+struct TileWalk {
+    uint64_t t, step, end;
+};
+__device__ __forceinline__ TileWalk tile_walk(uint64_t ntiles)
+{
+    const uint64_t g = gridDim.x, b = blockIdx.x;
+    if ((g & 7) == 0 && (ntiles & 7) == 0 && ntiles >= g) {
+        const uint64_t per = ntiles >> 3;
+        return TileWalk{(b & 7) * per + (b >> 3), g >> 3, ((b & 7) + 1) * per};
+    }
+    return TileWalk{b, g, ntiles};
+}
+
 __device__ __forceinline__ uint64_t ins0(uint64_t p, int q)
 {
     const uint64_t lo = p & ((1ull << q) - 1ull);

@@ -148,23 +148,6 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const DenseGeom &G)
     return t;
 }
 
-// The tiles a persistent workgroup takes: t, t + step, ... below end.  The
-// dispatcher hands consecutive workgroups to the 8 XCDs in turn, so when the
-// grid and the tile count split evenly, workgroup b (on XCD b % 8) walks the
-// b % 8-th eighth of the tiles and each XCD streams one contiguous range;
-// otherwise workgroup b takes b, b + grid, ...
-struct TileWalk {
-    uint64_t t, step, end;
-};
-__device__ __forceinline__ TileWalk tile_walk(uint64_t ntiles)
-{
-    const uint64_t g = gridDim.x, b = blockIdx.x;
-    if ((g & 7) == 0 && (ntiles & 7) == 0 && ntiles >= g) {
-        const uint64_t per = ntiles >> 3;
-        return TileWalk{(b & 7) * per + (b >> 3), g >> 3, ((b & 7) + 1) * per};
-    }
-    return TileWalk{b, g, ntiles};
-}
 
 template <typename R, int K, int GB, bool NT>
 __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__ a, uint64_t ntiles,

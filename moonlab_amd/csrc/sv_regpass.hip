@@ -357,17 +357,18 @@ __global__ __launch_bounds__(1 << (TB - RB), WPE) void k_reg_pass(cplx_t<R> *__r
     const int hi0 = tid >> 6;
     const BOp *pops = ops + P.first_op;
     cplx_t<R> v[NV], pre[NV];
-    uint64_t ti = blockIdx.x;
-    if (ti < ntiles) {
+    const TileWalk walk = tile_walk(ntiles);
+    uint64_t ti = walk.t;
+    if (ti < walk.end) {
         const uint64_t b0 = tile_base_c<TB>(ti, P) | lo;
 #pragma unroll
         for (int j = 0; j < NV; j++) pre[j] = ld<NT>(&a[b0 | dhi[hi0 + HSTEP * j]]);
     }
-    for (; ti < ntiles; ti += gridDim.x) {
+    for (; ti < walk.end; ti += walk.step) {
 #pragma unroll
         for (int j = 0; j < NV; j++) v[j] = pre[j];
-        const uint64_t nti = ti + gridDim.x;
-        if (nti < ntiles) {
+        const uint64_t nti = ti + walk.step;
+        if (nti < walk.end) {
             const uint64_t b1 = tile_base_c<TB>(nti, P) | lo;
 #pragma unroll
             for (int j = 0; j < NV; j++) pre[j] = ld<NT>(&a[b1 | dhi[hi0 + HSTEP * j]]);
