@@ -388,6 +388,7 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int 
             slot_off[j] = (uint32_t)((amp_bytes == 8 ? lds_slot<8>((int)roff(j)) : lds_slot<16>((int)roff(j))) *
                                      amp_bytes);
         memcpy(h.m, slot_off, sizeof slot_off);
+        const size_t hpos = out.size();
         out.push_back(h);
         for (const BOp *o : cur) {
             BOp b = *o;
@@ -447,8 +448,29 @@ void plan_reg_phases(const std::vector<BOp> &in0, int nbits, int amp_bytes, int 
                 const double r[2] = {m[6], m[7]};               // d3
                 memcpy(m, r, sizeof r);
             }
+            // two neighbouring real / RX-form rotations on different register
+            // bits (they commute) become one ARM_ROT2 op: one dispatch, both
+            // 2x2s in one pass over the register values
+            auto rot_form = [](const BOp &x) {
+                if (x.cg != 0) return -1;
+                if (x.pad >= ARM_MAT1_REAL && x.pad < ARM_MAT1_REAL + 4) return 0;
+                if (x.pad >= ARM_MAT1_RX && x.pad < ARM_MAT1_RX + 4) return 1;
+                return -1;
+            };
+            BOp &prev = out.back();
+            const int fb = rot_form(b);
+            if (out.size() > hpos + 1 && fb >= 0 && rot_form(prev) >= 0 && prev.lq != b.lq) {
+                const BOp &lo = prev.lq < b.lq ? prev : b, &hi = prev.lq < b.lq ? b : prev;
+                BOp pr = lo;
+                pr.pad = ARM_ROT2 + 4 * rot2_pair(lo.lq, hi.lq) + 2 * rot_form(lo) + rot_form(hi);
+                pr.lq2 = hi.lq;
+                memcpy(pr.m + 4, hi.m, 4 * sizeof(double));
+                prev = pr;
+                continue;
+            }
             out.push_back(b);
         }
+        out[hpos].lq = (int32_t)(out.size() - hpos - 1);
         rest.swap(later);
     }
 }

@@ -57,11 +57,20 @@ enum RegArm : int32_t {
     ARM_DIAG1_ONLY1 = 40,   // + P: diag(1, d1) on P, no controls
     ARM_DIAG11 = 44,        // + 4 P + Q (P < Q): diag(1, 1, 1, d3), no controls
     ARM_DIAG = 60,          // any diagonal (kmap, kg, jm)
+    ARM_ROT2 = 64,          // + 4 rot2_pair(P, Q) + 2 F + G: a form-F 2x2 on P (m[0..3]) and a
+                            // form-G 2x2 on Q (m[4..7]), P < Q, forms 0 real (RY, H) and 1 RX
 };
+
+// index of the register-bit pair P < Q (of 4 bits) among the ARM_ROT2 arms
+__host__ __device__ constexpr int rot2_pair(int p, int q)
+{
+    return p == 0 ? q - 1 : p == 1 ? q + 1 : 5;
+}
 // The entries of m[] an arm reads: MAT1, MAT1_MASKED and DIAG all eight as
 // built; MAT1_REAL m[0..3] = (u00, u01, u10, u11); MAT1_RX m[0..3] = (Re u00,
 // Im u01, Im u10, Re u11); DIAG1 / DIAG1_ONLY1 m[0..3] = (d0, d1); DIAG11
-// m[0..1] = d3.
+// m[0..1] = d3; ROT2 the first rotation's four in m[0..3], the second's in
+// m[4..7] (lq = P, lq2 = Q).
 
 // Register bits of a phase, ascending tile bits.
 struct RegLayout {

@@ -121,6 +121,52 @@ __device__ __forceinline__ void pp_mat1(const cplx_t<R> (&in)[1 << RB], cplx_t<R
     }
 }
 
+// one 2x2 of form F (0 real: m[0..3] = u00, u01, u10, u11; 1 RX: m[0..3] =
+// Re u00, Im u01, Im u10, Re u11) on the pair (a, b), the arithmetic of
+// pp_mat1's forms 1 and 2
+template <typename R, int F>
+__device__ __forceinline__ void rot_pair(const R (&m)[4], cplx_t<R> &a, cplx_t<R> &b)
+{
+    cplx_t<R> x, y;
+    if (F == 0) {
+        x.x = fmar(m[0], a.x, m[1] * b.x);
+        x.y = fmar(m[0], a.y, m[1] * b.y);
+        y.x = fmar(m[2], a.x, m[3] * b.x);
+        y.y = fmar(m[2], a.y, m[3] * b.y);
+    } else {
+        x.x = fmar(m[0], a.x, -(m[1] * b.y));
+        x.y = fmar(m[0], a.y, m[1] * b.x);
+        y.x = fmar(m[3], b.x, -(m[2] * a.y));
+        y.y = fmar(m[3], b.y, m[2] * a.x);
+    }
+    a = x;
+    b = y;
+}
+
+// two rotations on register bits P < Q in one pass over the values: form F
+// on P (m[0..3]), then form G on Q (m[4..7]); they commute, so the order is
+// the planner's choice
+template <typename R, int RB, int P, int Q, int F, int G>
+__device__ __forceinline__ void pp_rot2(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], const double *m)
+{
+    constexpr int NV = 1 << RB;
+    const R u[4] = {(R)m[0], (R)m[1], (R)m[2], (R)m[3]};
+    const R v[4] = {(R)m[4], (R)m[5], (R)m[6], (R)m[7]};
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (j & ((1 << P) | (1 << Q))) continue;
+        cplx_t<R> a = in[j], b = in[j | (1 << P)], c = in[j | (1 << Q)], d = in[j | (1 << P) | (1 << Q)];
+        rot_pair<R, F>(u, a, b);
+        rot_pair<R, F>(u, c, d);
+        rot_pair<R, G>(v, a, c);
+        rot_pair<R, G>(v, b, d);
+        out[j] = a;
+        out[j | (1 << P)] = b;
+        out[j | (1 << Q)] = c;
+        out[j | (1 << P) | (1 << Q)] = d;
+    }
+}
+
 // controlled X on register bit P for the values selected by jm (any controls)
 template <typename R, int RB, int P>
 __device__ __forceinline__ void pp_xperm(const cplx_t<R> (&in)[1 << RB], cplx_t<R> (&out)[1 << RB], uint32_t jm)
@@ -265,6 +311,16 @@ __device__ __forceinline__ void pp_op(const cplx_t<R> (&in)[1 << RB], cplx_t<R> 
         if constexpr (RB > 3 || (P < 3 && Q < 3)) CALL<R, RB, P, Q>(in, out);      \
         else pp_copy<R, RB>(in, out);                                              \
         break;
+#define MLX_PP_ROT2_FG(P, Q, F, G)                                                  \
+    case ARM_ROT2 + 4 * rot2_pair(P, Q) + 2 * F + G:                               \
+        if constexpr (RB > 3 || Q < 3) pp_rot2<R, RB, P, Q, F, G>(in, out, full());  \
+        else pp_copy<R, RB>(in, out);                                              \
+        break;
+#define MLX_PP_ROT2(P, Q)                                                          \
+    MLX_PP_ROT2_FG(P, Q, 0, 0)                                                     \
+    MLX_PP_ROT2_FG(P, Q, 0, 1)                                                     \
+    MLX_PP_ROT2_FG(P, Q, 1, 0)                                                     \
+    MLX_PP_ROT2_FG(P, Q, 1, 1)
 #define MLX_PP_PAIR_OP(ARM, P, Q, CALL)                                            \
     case ARM + 4 * P + Q:                                                          \
         if constexpr (RB > 3 || (P < 3 && Q < 3)) CALL<R, RB, P, Q>(in, out, m);   \
@@ -296,6 +352,12 @@ __device__ __forceinline__ void pp_op(const cplx_t<R> (&in)[1 << RB], cplx_t<R> 
         MLX_PP_PAIR_OP(ARM_DIAG11, 1, 2, pp_diag11)
         MLX_PP_PAIR_OP(ARM_DIAG11, 1, 3, pp_diag11)
         MLX_PP_PAIR_OP(ARM_DIAG11, 2, 3, pp_diag11)
+        MLX_PP_ROT2(0, 1)
+        MLX_PP_ROT2(0, 2)
+        MLX_PP_ROT2(0, 3)
+        MLX_PP_ROT2(1, 2)
+        MLX_PP_ROT2(1, 3)
+        MLX_PP_ROT2(2, 3)
     default: {   // ARM_DIAG
         const int kg = global_bit(op->lq, base) | (global_bit(op->lq2, base) << 1);
         pp_diag<R, RB>(in, out, *op, jm, op->cl, __builtin_amdgcn_readfirstlane(kg));
@@ -312,6 +374,8 @@ __device__ __forceinline__ void pp_op(const cplx_t<R> (&in)[1 << RB], cplx_t<R> 
 #undef MLX_PP_ONLY1
 #undef MLX_PP_PAIR
 #undef MLX_PP_PAIR_OP
+#undef MLX_PP_ROT2_FG
+#undef MLX_PP_ROT2
 }
 
 // One pass.  A tile enters and leaves in the IO layout, whose register bits

@@ -17,6 +17,8 @@
 #include "batch_plan.h"
 extern "C" {
 #include "sv_oracle.h"
+
+static long g_rot2_ops = 0;   // ARM_ROT2 ops run (paired rotations)
 }
 using cd = std::complex<double>;
 static uint64_t ins0(uint64_t x, int q) { return ((x >> q) << (q + 1)) | (x & ((1ull << q) - 1)); }
@@ -159,6 +161,18 @@ static void reg_tile(std::vector<cd> &tile, int TB, int RB, const mlx::BOp *ops,
                         const int kk = kg | (int)((op.cl >> (2 * j)) & 3u);
                         w[j] *= cd(op.m[2 * kk], op.m[2 * kk + 1]);
                     }
+                } else if (arm >= mlx::ARM_ROT2 && arm < mlx::ARM_ROT2 + 24) {   // two rotations, P < Q
+                    static const int pp[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+                    const int pi = (arm - mlx::ARM_ROT2) / 4, F = ((arm - mlx::ARM_ROT2) >> 1) & 1, G = (arm - mlx::ARM_ROT2) & 1;
+                    const int P = pp[pi][0], Q = pp[pi][1];
+                    if (op.kind != mlx::BOP_MAT1 || P != op.lq || Q != op.lq2 || Q >= RB || mlx::rot2_pair(P, Q) != pi) { printf("ROT2 arm mismatch\n"); exit(1); }
+                    auto rot = [&](int form, const double *m, cd &a, cd &b) {
+                        if (form == 0) { cd x = m[0] * a + m[1] * b, y = m[2] * a + m[3] * b; a = x; b = y; }
+                        else { cd x = m[0] * a + cd(0, m[1]) * b, y = cd(0, m[2]) * a + m[3] * b; a = x; b = y; }
+                    };
+                    if (tid == 0) g_rot2_ops++;
+                    for (int j = 0; j < NV; j++) if (!(j & (1 << P))) rot(F, op.m, w[j], w[j | (1 << P)]);
+                    for (int j = 0; j < NV; j++) if (!(j & (1 << Q))) rot(G, op.m + 4, w[j], w[j | (1 << Q)]);
                 } else { printf("unknown arm %d\n", arm); exit(1); }
             }
         }
@@ -322,6 +336,6 @@ int main() {
         if (err > 1e-12) { printf("LOOKAHEAD PLAN MISMATCH err=%g\n", err); return 1; }
         reg_checked++;
     }
-    printf("ok: %d plans checked, %d segment executions, %d register-phase plans, %d layout changes\n", checked,
-           segs_seen, reg_checked, relayouts);
+    printf("ok: %d plans checked, %d segment executions, %d register-phase plans, %d layout changes, "
+           "%ld paired rotations\n", checked, segs_seen, reg_checked, relayouts, g_rot2_ops);
 }

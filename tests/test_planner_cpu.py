@@ -28,3 +28,5 @@ def test_planned_passes_reproduce_circuits(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok:"), r.stdout
+    # the register-phase programs include paired rotations (ARM_ROT2)
+    assert int(r.stdout.split(",")[-1].split()[0]) > 0, r.stdout
