@@ -571,11 +571,11 @@ def main() -> int:
     ap.add_argument("--time-budget", type=float, default=540.0,
                     help="seconds the whole default run may take: legs past it are skipped and named")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--workload", choices=("sweep", "fusion", "blocked", "batch"), default="sweep",
+    ap.add_argument("--workload", choices=("sweep", "fusion", "blocked", "batch", "serving"), default="sweep",
                     help="sweep: the 30-qubit gate sweep (with the batch and serving legs); "
                          "fusion: 32q complex64 and 33q complex128 SU(2)+CZ depth 60 on fused MFMA blocks; "
                          "blocked: cache-blocked register-phase passes; "
-                         "batch: the batch leg alone (for profiler runs)")
+                         "batch: the batch leg alone; serving: the served stream alone (both for profiler runs)")
     ap.add_argument("--fusion-depth", type=int, default=60)
     ap.add_argument("--fusion-cases", default="32:c64,33:c128")
     ap.add_argument("--probe", action="store_true",
@@ -609,6 +609,19 @@ def main() -> int:
                          "unit": "GB/s", "frac": round(c128["register_TB_per_s_per_pass"] * 1e3 / HBM_PEAK_GBS, 4),
                          "traffic": None},
             "cases": res}), flush=True)
+        return 0
+
+    if args.workload == "serving":
+        t0 = time.perf_counter()
+        sv = run_serving_workload(m, args.serve_count, args.serve_rate, False)
+        wall = time.perf_counter() - t0
+        print(json.dumps({
+            "metric": "circuits/s, served request stream (co-batching + cache-blocked large requests)",
+            "value": sv["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": round(sv["makespan_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64/f32", "data": "synthetic",
+            "config": {"workload": f"{args.serve_count} Poisson requests at {args.serve_rate:g}/s"},
+            "served": sv, "bench_seconds": round(wall, 1)}), flush=True)
         return 0
 
     if args.workload == "batch":
