@@ -501,6 +501,30 @@ def test_blocked_engines_vs_oracle(eng, oracle, engine, tile_bits):
 
 
 @pytest.mark.gpu
+def test_blocked_random_layers_vs_oracle(eng, oracle):
+    """The serving stream's circuit family (rotation per qubit + CNOT
+    brickwork) on the default register-phase engine: its phases are mostly
+    paired rotations (ARM_ROT2) and CNOT exchanges.  complex128 within 1e-12
+    of the oracle.  complex64: the direction of the state by fidelity after
+    normalising, and the norm separately -- the gate matrices rounded to
+    float32 are unitary only to ~1e-7 (c^2 + s^2 != 1, 2 * fl(1/sqrt 2)^2 < 1),
+    so the norm drifts by ~1e-6 over a few hundred gates, by design of the
+    precision, while the direction stays within ~1e-12."""
+    from moonlab_amd import workloads
+    for n, depth in ((14, 12), (20, 8), (23, 6)):
+        gl = workloads.random_layer_circuit(n, depth, 700 + n, h_layer=True)
+        want = oracle.run(n, gl)
+        with eng.StateVector(n) as s:
+            got = s.apply(gl, blocked=True).amplitudes()
+        assert np.max(np.abs(got - want)) < TOL, n
+        with eng.StateVector(n, eng.C64) as s:
+            got = s.apply(gl, blocked=True).amplitudes()
+        nrm = np.vdot(got, got).real
+        assert abs(nrm - 1) < 2e-5, (n, nrm)
+        assert abs(np.vdot(want, got)) ** 2 / nrm > 1 - 1e-8, n
+
+
+@pytest.mark.gpu
 def test_plugin_runs_in_reference_scheduler(eng):
     """Registered into the reference's own scheduler (compiled reference C),
     a job set to "mi355x" returns the same outcomes as the reference's
