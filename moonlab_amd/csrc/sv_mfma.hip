@@ -6,7 +6,9 @@
 //     Y (2^(k+1) x G) = A (2^(k+1) x 2^(k+1)) * X (2^(k+1) x G),
 //     A[2r+a][2c+b] = {Mr, -Mi; Mi, Mr}[a][b](r, c),  X[2c+b][g] = (re, im)_b of x_c(g),
 // it maps onto v_mfma_{f32,f64}_16x16x4 with FP32 (complex64) or FP64
-// (complex128) accumulation:
+// (complex128) accumulation.  For k >= 4 the same product runs as three
+// real 2^k x 2^k GEMMs (the 3M method, struct Block below), 3/4 of the
+// real-embedding MFMAs:
 //   * A (the block's matrix) stays in VGPRs for the whole launch;
 //   * a 256-thread workgroup stages a tile of 64 * 2^gb groups in LDS (about
 //     32 KiB): the tile set is Q plus the 6 + gb lowest other qubits, so
@@ -148,6 +150,122 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const DenseGeom &G)
     return t;
 }
 
+// The contraction of one 16-group column of a fused block, shared by both
+// kernels.  k >= 4 blocks run as three real products (the 3M method):
+//     P1 = Mr xr,  P2 = Mi xi,  P3 = (Mr + Mi)(xr + xi),
+//     Re y = P1 - P2,  Im y = P3 - P1 - P2,
+// each a 2^k x 2^k real GEMM: 3 * (2^k / 16) * (2^k / 4) MFMAs per 16
+// groups, three quarters of the real-embedding form's (2^(k+1))^2 / 64, and
+// the B operand is one complex LDS read per k-step instead of two halves.
+// k = 3 (8 rows, less than one 16-row tile) keeps the real embedding
+// A[2r+a][2c+b] = {Mr, -Mi; Mi, Mr}[a][b](r, c).
+// Output o of a lane is block row out_row(o) of its group.
+template <typename R, int K>
+struct Block {
+    static constexpr int D = 1 << K;
+    static constexpr bool M3 = K >= 4;
+    static constexpr int NP = M3 ? 3 : 1;                      // real products
+    static constexpr int NRT = (M3 ? D : 2 * D) / 16;          // 16-row tiles
+    static constexpr int NS = (M3 ? D : 2 * D) / 4;            // k-steps
+    static constexpr int NOUT = M3 ? 4 * NRT : 2 * NRT;        // complex outputs per lane
+    using AccT = Acc<R>;
+    R Am[NP][NRT][NS];
+
+    // Block row of accumulator v of lane group q = l / 16 in a 16-row tile
+    // of the 3M products: (v & 1) + 2 q + 8 (v / 2), so one store (fixed v)
+    // covers rows that differ in bits 1..2, as in the real-embedding form.
+    static __device__ __forceinline__ int m3_row(int v, int q) { return (v & 1) + 2 * q + 8 * (v >> 1); }
+    // block row that A row i carries: accumulator v holds D[4 q + v] for
+    // FP32 and D[4 v + q] for FP64
+    static __device__ __forceinline__ int m3_arow(int i)
+    {
+        if constexpr (sizeof(R) == 4) return m3_row(i & 3, i >> 2);
+        else return m3_row(i >> 2, i & 3);
+    }
+
+    // A operands: rows 16 rt + (a_row or m3_arow)(l % 16), columns 4 s + l / 16
+    __device__ __forceinline__ void load(const cplx_t<R> *__restrict__ mat, int lane)
+    {
+#pragma unroll
+        for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int col = 4 * s + (lane >> 4);
+                if constexpr (M3) {
+                    const int row = 16 * rt + m3_arow(lane & 15);
+                    const cplx_t<R> m = mat[row * D + col];
+                    Am[0][rt][s] = m.x;
+                    Am[1][rt][s] = m.y;
+                    Am[2][rt][s] = m.x + m.y;
+                } else {
+                    const int row = 16 * rt + AccT::a_row(lane & 15);
+                    const cplx_t<R> m = mat[(row >> 1) * D + (col >> 1)];
+                    const int ra = row & 1, cb = col & 1;
+                    Am[0][rt][s] = ra == cb ? m.x : (ra == 0 ? -m.y : m.y);
+                }
+            }
+    }
+    // real embedding: accumulator v of row tile rt holds real row
+    // 16 rt + 4 (l / 16) + v (the FP64 A rows are permuted by a_row to give
+    // the FP32 order), i.e. block row 8 rt + 2 (l / 16) + v / 2
+    static __device__ __forceinline__ int out_row(int o, int lane)
+    {
+        if constexpr (M3) return 16 * (o >> 2) + m3_row(o & 3, lane >> 4);
+        else return 8 * (o >> 1) + 2 * (lane >> 4) + (o & 1);
+    }
+    // get(c): amplitude c of this lane's group; put(o, y): output o
+    template <class Get, class Put>
+    __device__ __forceinline__ void apply(int lane, Get get, Put put) const
+    {
+        if constexpr (M3) {
+            R br[NS], bi[NS], bs[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const cplx_t<R> x = get(4 * s + (lane >> 4));
+                br[s] = x.x;
+                bi[s] = x.y;
+                bs[s] = x.x + x.y;
+            }
+            typename AccT::type p1[NRT], p2[NRT], p3[NRT];
+#pragma unroll
+            for (int rt = 0; rt < NRT; rt++) {
+                p1[rt] = p2[rt] = p3[rt] = typename AccT::type{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    p1[rt] = AccT::mma(Am[0][rt][s], br[s], p1[rt]);
+                    p2[rt] = AccT::mma(Am[1][rt][s], bi[s], p2[rt]);
+                    p3[rt] = AccT::mma(Am[2][rt][s], bs[s], p3[rt]);
+                }
+            }
+#pragma unroll
+            for (int rt = 0; rt < NRT; rt++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    put(4 * rt + v, cplx_t<R>{p1[rt][v] - p2[rt][v], p3[rt][v] - p1[rt][v] - p2[rt][v]});
+        } else {
+            R B[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int kk = 4 * s + (lane >> 4);
+                const cplx_t<R> x = get(kk >> 1);
+                B[s] = (kk & 1) ? x.y : x.x;
+            }
+            typename AccT::type acc[NRT];
+#pragma unroll
+            for (int rt = 0; rt < NRT; rt++) {
+                acc[rt] = typename AccT::type{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < NS; s++) acc[rt] = AccT::mma(Am[0][rt][s], B[s], acc[rt]);
+            }
+#pragma unroll
+            for (int rt = 0; rt < NRT; rt++) {
+                put(2 * rt, cplx_t<R>{acc[rt][0], acc[rt][1]});
+                put(2 * rt + 1, cplx_t<R>{acc[rt][2], acc[rt][3]});
+            }
+        }
+    }
+};
+
 
 template <typename R, int K, int GB, bool NT>
 __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__ a, uint64_t ntiles,
@@ -155,14 +273,11 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
 {
     constexpr int D = 1 << K;
     constexpr int GM = 1 << GB;         // 64-group sets per tile
-    constexpr int NRT = (2 * D) / 16;   // row tiles
-    constexpr int NS = (2 * D) / 4;     // k-steps
     constexpr int T = 64 * GM * D;      // amplitudes per tile
     constexpr int A = Slot<R>::A;       // amplitudes per 16-byte slot
     constexpr int PER = T / A / kThreads;   // slots each thread moves per tile
     constexpr int HSTEP = kThreads * A / 64;
     static_assert(T <= 4096 && PER >= 1 && T / 64 <= 64, "tile shape");
-    using AccT = Acc<R>;
     using SlotT = typename Slot<R>::type;
     __shared__ cplx_t<R> lds[T + T / 32];
     __shared__ uint64_t dlo[64], dhi[64];
@@ -186,24 +301,14 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
         offg_hi[tid - 224] = G.offg_hi[tid - 224];
     }
 
-    // A operands: rows 16 rt + a_row(l % 16), columns 4 s + l / 16 of the real matrix
-    R Am[NRT][NS];
-#pragma unroll
-    for (int rt = 0; rt < NRT; rt++)
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const int row = 16 * rt + AccT::a_row(lane & 15), col = 4 * s + (lane >> 4);
-            const cplx_t<R> m = mat[(row >> 1) * D + (col >> 1)];
-            const int ra = row & 1, cb = col & 1;
-            Am[rt][s] = ra == cb ? m.x : (ra == 0 ? -m.y : m.y);
-        }
+    Block<R, K> blk;
+    blk.load(mat, lane);
     __syncthreads();
 
     // slot s = tid + kThreads i holds tile amplitudes A s .. A s + A - 1:
     // (A s) % 64 = A (tid % (64 / A)) and (A s) / 64 = tid / (64 / A) + HSTEP i
     const uint64_t lo = dlo[A * (tid & (64 / A - 1))];
     const int hi0 = tid / (64 / A);
-    const R *ldsr = (const R *)lds;
 
     // software pipeline: the walk's next tile is in flight from global
     // memory while tile t is contracted and written back
@@ -231,25 +336,9 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
         for (int j = 0; j < GM; j++) {
             const int g = 64 * j + 16 * wave + (lane & 15);
             const int go = offg_lo[g & 63] | offg_hi[g >> 6];
-            R B[NS];
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int kk = 4 * s + (lane >> 4);
-                B[s] = ldsr[2 * padded(offq[kk >> 1] | go) + (kk & 1)];
-            }
-            typename AccT::type acc[NRT];
-#pragma unroll
-            for (int rt = 0; rt < NRT; rt++) {
-                acc[rt] = typename AccT::type{0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < NS; s++) acc[rt] = AccT::mma(Am[rt][s], B[s], acc[rt]);
-            }
-#pragma unroll
-            for (int rt = 0; rt < NRT; rt++) {
-                const int r0 = 8 * rt + 2 * (lane >> 4);
-                lds[padded(offq[r0] | go)] = cplx_t<R>{acc[rt][0], acc[rt][1]};
-                lds[padded(offq[r0 + 1] | go)] = cplx_t<R>{acc[rt][2], acc[rt][3]};
-            }
+            blk.apply(
+                lane, [&](int c) { return lds[padded(offq[c] | go)]; },
+                [&](int o, const cplx_t<R> &y) { lds[padded(offq[Block<R, K>::out_row(o, lane)] | go)] = y; });
         }
         lds_barrier();
         const uint64_t base = tile_base<K + 6 + GB>(t, G) | lo;
@@ -315,16 +404,13 @@ __device__ __forceinline__ void ring_loop(cplx_t<R> *__restrict__ a, uint64_t nt
                                           const uint16_t *__restrict__ offq, const uint16_t *__restrict__ offg_lo,
                                           const uint16_t *__restrict__ offg_hi, const uint64_t *__restrict__ dlo,
                                           const uint64_t *__restrict__ dhi,
-                                          const R (&Am)[(2 << K) / 16][(2 << K) / 4], const uint64_t (&off)[8],
-                                          const uint64_t (&gq)[(2 << K) / 16][2])
+                                          const Block<R, K> &blk, const uint64_t (&off)[8],
+                                          const uint64_t (&gq)[Block<R, K>::NOUT])
 {
     constexpr int GM = 1 << GB;
-    constexpr int NRT = (2 << K) / 16;
-    constexpr int NS = (2 << K) / 4;
     constexpr int PER = 8;
-    constexpr int ST = 2 * NRT * GM;                                    // stores per thread per tile
+    constexpr int ST = Block<R, K>::NOUT * GM;                          // stores per thread per tile
     constexpr int AHEAD = 3 * PER + 3 * ST < 63 ? 3 * PER + 3 * ST : 63;   // memory ops issued after a tile's loads
-    using AccT = Acc<R>;
     using SlotT = typename Slot<R>::type;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto buffer = [&](auto bc) -> SlotT * {
@@ -364,31 +450,16 @@ __device__ __forceinline__ void ring_loop(cplx_t<R> *__restrict__ a, uint64_t nt
         }
         ring_barrier();
         const uint64_t base = tile_base<K + 6 + GB>(t, G);
-        const R *ldsr = (const R *)buf;
+        const cplx_t<R> *bufc = (const cplx_t<R> *)buf;
 #pragma unroll 1
         for (int j = 0; j < GM; j++) {
             const int g = 64 * j + 16 * wave + (lane & 15);
             const int go = offg_lo[g & 63] | offg_hi[g >> 6];
-            R Bv[NS];
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int kk = 4 * s + (lane >> 4);
-                Bv[s] = ldsr[2 * ring_pos<R>(offq[kk >> 1] | go) + (kk & 1)];
-            }
             const uint64_t gg = base | dlo[go & 63] | dhi[go >> 6];
-            typename AccT::type acc[NRT];
-#pragma unroll
-            for (int rt = 0; rt < NRT; rt++) {
-                acc[rt] = typename AccT::type{0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < NS; s++) acc[rt] = AccT::mma(Am[rt][s], Bv[s], acc[rt]);
-            }
-            // results straight from the accumulators to HBM: rows r0, r0 + 1 of group g
-#pragma unroll
-            for (int rt = 0; rt < NRT; rt++) {
-                st<NT>(&a[gg | gq[rt][0]], cplx_t<R>{acc[rt][0], acc[rt][1]});
-                st<NT>(&a[gg | gq[rt][1]], cplx_t<R>{acc[rt][2], acc[rt][3]});
-            }
+            // results straight from the accumulators to HBM
+            blk.apply(
+                lane, [&](int c) { return bufc[ring_pos<R>(offq[c] | go)]; },
+                [&](int o, const cplx_t<R> &y) { st<NT>(&a[gg | gq[o]], y); });
         }
         ring_barrier();   // every wave has read this buffer before a load refills it
         t += step;
@@ -405,14 +476,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_dense_mfma_ring(cplx_t<R> *__re
 {
     constexpr int D = 1 << K;
     constexpr int GM = 1 << GB;
-    constexpr int NRT = (2 * D) / 16;
-    constexpr int NS = (2 * D) / 4;
     constexpr int T = 64 * GM * D;
     constexpr int A = Slot<R>::A;
     constexpr int NSLOT = T / A;             // 16-byte slots per tile
     constexpr int PER = NSLOT / kThreads;    // slots each thread loads and stores per tile
     static_assert(NSLOT == 2048 && PER == 8, "32 KiB tiles");
-    using AccT = Acc<R>;
     using SlotT = typename Slot<R>::type;
     // one variable per ring buffer, each addressed with a constant: the
     // compiler then sees that reading one buffer does not depend on the
@@ -439,16 +507,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_dense_mfma_ring(cplx_t<R> *__re
     } else if (tid < 232) {
         offg_hi[tid - 224] = G.offg_hi[tid - 224];
     }
-    R Am[NRT][NS];
-#pragma unroll
-    for (int rt = 0; rt < NRT; rt++)
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const int row = 16 * rt + AccT::a_row(lane & 15), col = 4 * s + (lane >> 4);
-            const cplx_t<R> m = mat[(row >> 1) * D + (col >> 1)];
-            const int ra = row & 1, cb = col & 1;
-            Am[rt][s] = ra == cb ? m.x : (ra == 0 ? -m.y : m.y);
-        }
+    Block<R, K> blk;
+    blk.load(mat, lane);
     __syncthreads();
 
     // load / store i of this thread: LDS slot 64 (wave + 4 i) + lane, tile slot swz of it
@@ -458,16 +518,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_dense_mfma_ring(cplx_t<R> *__re
         const int l = A * ring_swz<R>(64 * (wave + 4 * i) + lane);
         off[i] = dlo[l & 63] | dhi[l >> 6];
     }
-    // global offsets of the result rows this lane writes: r0 = 8 rt + 2 (lane / 16) and r0 + 1
-    uint64_t gq[NRT][2];
+    // global offsets of the result rows this lane writes
+    uint64_t gq[Block<R, K>::NOUT];
 #pragma unroll
-    for (int rt = 0; rt < NRT; rt++)
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int lq = offq[8 * rt + 2 * (lane >> 4) + h];
-            gq[rt][h] = dlo[lq & 63] | dhi[lq >> 6];
-        }
-    ring_loop<R, K, GB, NT>(a, ntiles, G, ring0, ring1, ring2, ring3, offq, offg_lo, offg_hi, dlo, dhi, Am, off, gq);
+    for (int o = 0; o < Block<R, K>::NOUT; o++) {
+        const int lq = offq[Block<R, K>::out_row(o, lane)];
+        gq[o] = dlo[lq & 63] | dhi[lq >> 6];
+    }
+    ring_loop<R, K, GB, NT>(a, ntiles, G, ring0, ring1, ring2, ring3, offq, offg_lo, offg_hi, dlo, dhi, blk, off, gq);
 }
 
 // Tile set = Q plus the 6 + gb lowest other qubits; local indices follow the
