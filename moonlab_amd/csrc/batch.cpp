@@ -38,8 +38,7 @@ namespace mlx {
 
 hipError_t launch_tile_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int threads,
                             int tile_bits, hipStream_t st);
-hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int cus, int reg_bits,
-                           hipStream_t st);
+hipError_t launch_reg_pass(void *amps, int prec, int n, const BPass &P, const BOp *d_ops, int reg_bits, hipStream_t st);
 
 hipError_t launch_batch(const BInst *insts, const int32_t *order, int ninst, int *counter, const BPass *passes,
                         const BOp *ops, double2 *scratch, uint64_t slot_amps, double *tile_scratch,
@@ -1081,7 +1080,7 @@ moonlab_hip_status_t moonlab_hip_apply_gates_blocked(moonlab_hip_state_t *s, con
     void *amps = moonlab_hip_state_device_ptr(s);
     const int lds_tile = tune.blocked_tile_bits == 0 || tune.blocked_tile_bits > 12 ? 12 : tune.blocked_tile_bits;
     for (size_t i = 0; e == hipSuccess && i < passes.size(); i++)
-        e = reg ? launch_reg_pass(amps, prec, n, passes[i], d_ops, cus, reg_bits, st)
+        e = reg ? launch_reg_pass(amps, prec, n, passes[i], d_ops, reg_bits, st)
                 : launch_tile_pass(amps, prec, n, passes[i], d_ops, cus, tune.blocked_threads, lds_tile, st);
     if (d_ops) hipFreeAsync(d_ops, st);
     if (e == hipErrorOutOfMemory) return MOONLAB_HIP_ERR_OUT_OF_MEMORY;
