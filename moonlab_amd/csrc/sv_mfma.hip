@@ -19,7 +19,8 @@
 //   * each wave contracts 16 groups at a time: (2^(k+1)/16) row tiles x
 //     (2^(k+1)/4) k-steps of 16x16x4 MFMAs; it reads and writes only its own
 //     groups, so no barrier separates the contraction from the write-back;
-//   * the next tile's global loads are issued before the current tile is
+//   * a workgroup streams a few tiles (4 complex64, 32 complex128) and exits;
+//     the next tile's global loads are issued before the current tile is
 //     contracted, and the barriers order LDS only, so loads, MFMAs and the
 //     previous tile's stores overlap.
 // Operand layout of 16x16x4 (lane l): A[l % 16][l / 16], B[l / 16][l % 16];
@@ -27,15 +28,8 @@
 // D[4 v + l / 16][l % 16] for FP64.  The FP64 path loads A with its rows
 // permuted (A row i = real row 4 (i % 4) + i / 4), which puts both results in
 // the FP32 order, so one write-back serves both.
-// Complex128 5-qubit blocks clear of qubits 0..2 run k_dense_mfma_ring
-// instead (direct global->LDS loads into a 4-tile ring, results stored from
-// the accumulators; described at the kernel).
 #include "sv_device.h"
 #include "sv_internal.h"
-
-#include <atomic>
-#include <type_traits>
-#include <vector>
 
 namespace mlx {
 
@@ -127,7 +121,6 @@ __device__ __forceinline__ float4 get_slot(const float2 *l, int i)
 struct DenseGeom {
     int32_t  tb;           // tile bits = k + 6 + gb (<= 12)
     int32_t  gb;
-    int32_t  plain;        // 1: workgroup b takes tiles b, b + grid, ... (no XCD remap)
     int8_t   tq[16];       // tile-set qubits, ascending
     uint16_t offq[32];     // local index of amplitude r of a group, r < 2^k
     uint16_t offg_lo[64];  // local index contributed by group bits 0..5
@@ -314,7 +307,7 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
     // software pipeline: the walk's next tile is in flight from global
     // memory while tile t is contracted and written back
     SlotT pre[PER];
-    const TileWalk walk = G.plain ? TileWalk{blockIdx.x, gridDim.x, ntiles} : tile_walk(ntiles);
+    const TileWalk walk{blockIdx.x, gridDim.x, ntiles};   // tiles b, b + grid, ...
     uint64_t t = walk.t;
     if (t < walk.end) {
         const uint64_t base = tile_base<K + 6 + GB>(t, G) | lo;
@@ -350,185 +343,6 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
     }
 }
 
-// Ring variant (complex128, k = 5): tiles stream into a ring of kRing 32 KiB
-// LDS buffers with direct global->LDS loads (global_load_lds_dwordx4),
-// kRing - 1 tiles ahead of the contraction, one 256-thread workgroup per CU,
-// so 96 KiB per CU are in flight all the time and no VGPRs hold tiles in
-// flight.  The results go from the accumulators straight to HBM (no LDS
-// write-back, one barrier less per tile).  A wave's load instruction fills
-// 64 consecutive LDS slots, so LDS slot p holds tile slot swz(p), an XOR
-// within 32-amplitude rows that keeps the strided operand reads on distinct
-// banks as the padding above does; each half-wave's slots are still one
-// contiguous 512-byte run in HBM.
-constexpr int kRing = 4;
-
-template <typename R>
-__device__ __forceinline__ int ring_swz(int p)
-{
-    constexpr int SH = Slot<R>::A == 1 ? 5 : 4;   // 32 amplitudes per row
-    return p ^ ((p >> SH) & 15);
-}
-// complex index in a ring buffer of tile amplitude l
-template <typename R>
-__device__ __forceinline__ int ring_pos(int l)
-{
-    if constexpr (Slot<R>::A == 1) return ring_swz<R>(l);
-    else return 2 * ring_swz<R>(l >> 1) + (l & 1);
-}
-
-// s_waitcnt on vmcnt only (gfx9 encoding: vmcnt bits 3:0 and 15:14)
-template <int N>
-__device__ __forceinline__ void wait_vmcnt()
-{
-    static_assert(N >= 0 && N < 64, "vmcnt");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-// Workgroup barrier for the ring loop: LDS operations of this wave complete
-// (lgkmcnt), then s_barrier.  The fence-based lds_barrier() would also wait
-// for every direct global->LDS load in flight (they count as LDS writes),
-// draining the ring at every tile; the loop waits for its own tile's loads
-// with wait_vmcnt before this barrier instead.
-__device__ __forceinline__ void ring_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
-// The persistent loop of k_dense_mfma_ring.  Every LDS array it touches is a
-// separate __restrict__ parameter: the alias scopes this gives the accesses
-// let the compiler see that reading one ring buffer (or the index tables)
-// does not depend on the direct loads still filling the other buffers, so it
-// waits only where the loop asks it to.
-template <typename R, int K, int GB, bool NT>
-__device__ __forceinline__ void ring_loop(cplx_t<R> *__restrict__ a, uint64_t ntiles, const DenseGeom &G,
-                                          typename Slot<R>::type *__restrict__ ring0,
-                                          typename Slot<R>::type *__restrict__ ring1,
-                                          typename Slot<R>::type *__restrict__ ring2,
-                                          typename Slot<R>::type *__restrict__ ring3,
-                                          const uint16_t *__restrict__ offq, const uint16_t *__restrict__ offg_lo,
-                                          const uint16_t *__restrict__ offg_hi, const uint64_t *__restrict__ dlo,
-                                          const uint64_t *__restrict__ dhi,
-                                          const Block<R, K> &blk, const uint64_t (&off)[8],
-                                          const uint64_t (&gq)[Block<R, K>::NOUT])
-{
-    constexpr int GM = 1 << GB;
-    constexpr int PER = 8;
-    constexpr int ST = Block<R, K>::NOUT * GM;                          // stores per thread per tile
-    constexpr int AHEAD = 3 * PER + 3 * ST < 63 ? 3 * PER + 3 * ST : 63;   // memory ops issued after a tile's loads
-    using SlotT = typename Slot<R>::type;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    auto buffer = [&](auto bc) -> SlotT * {
-        constexpr int b = decltype(bc)::value;
-        if constexpr (b == 0) return ring0;
-        else if constexpr (b == 1) return ring1;
-        else if constexpr (b == 2) return ring2;
-        else return ring3;
-    };
-    auto issue = [&](uint64_t t, SlotT *buf) {
-        const uint64_t base = tile_base<K + 6 + GB>(t, G);
-#pragma unroll
-        for (int i = 0; i < PER; i++)
-            __builtin_amdgcn_global_load_lds((const void *)&a[base | off[i]],
-                                             (__attribute__((address_space(3))) void *)&buf[64 * (wave + 4 * i)], 16,
-                                             0, 0);
-    };
-    const TileWalk walk = tile_walk(ntiles);
-    const uint64_t step = walk.step, end = walk.end;
-    uint64_t t = walk.t;
-    if (t < end) issue(t, ring0);
-    if (t + step < end) issue(t + step, ring1);
-    if (t + 2 * step < end) issue(t + 2 * step, ring2);
-
-    // tile t from buffer B; the load of tile t + 3 step goes to buffer B - 1,
-    // which every wave finished reading before the previous tile's last barrier
-    auto tile = [&](auto bc) -> bool {
-        constexpr int B = decltype(bc)::value;
-        if (t >= end) return false;
-        const SlotT *buf = buffer(bc);
-        const uint64_t ahead = t + (kRing - 1) * step;
-        if (ahead < end) {
-            issue(ahead, buffer(std::integral_constant<int, (B + kRing - 1) % kRing>{}));
-            wait_vmcnt<AHEAD>();   // tile t's loads are done (vmcnt retires in issue order)
-        } else {
-            wait_vmcnt<0>();
-        }
-        ring_barrier();
-        const uint64_t base = tile_base<K + 6 + GB>(t, G);
-        const cplx_t<R> *bufc = (const cplx_t<R> *)buf;
-#pragma unroll 1
-        for (int j = 0; j < GM; j++) {
-            const int g = 64 * j + 16 * wave + (lane & 15);
-            const int go = offg_lo[g & 63] | offg_hi[g >> 6];
-            const uint64_t gg = base | dlo[go & 63] | dhi[go >> 6];
-            // results straight from the accumulators to HBM
-            blk.apply(
-                lane, [&](int c) { return bufc[ring_pos<R>(offq[c] | go)]; },
-                [&](int o, const cplx_t<R> &y) { st<NT>(&a[gg | gq[o]], y); });
-        }
-        ring_barrier();   // every wave has read this buffer before a load refills it
-        t += step;
-        return true;
-    };
-    while (tile(std::integral_constant<int, 0>{}) && tile(std::integral_constant<int, 1>{}) &&
-           tile(std::integral_constant<int, 2>{}) && tile(std::integral_constant<int, 3>{})) {
-    }
-}
-
-template <typename R, int K, int GB, bool NT>
-__global__ __launch_bounds__(kThreads, 1) void k_dense_mfma_ring(cplx_t<R> *__restrict__ a, uint64_t ntiles,
-                                                                 DenseGeom G, const cplx_t<R> *__restrict__ mat)
-{
-    constexpr int D = 1 << K;
-    constexpr int GM = 1 << GB;
-    constexpr int T = 64 * GM * D;
-    constexpr int A = Slot<R>::A;
-    constexpr int NSLOT = T / A;             // 16-byte slots per tile
-    constexpr int PER = NSLOT / kThreads;    // slots each thread loads and stores per tile
-    static_assert(NSLOT == 2048 && PER == 8, "32 KiB tiles");
-    using SlotT = typename Slot<R>::type;
-    // one variable per ring buffer, each addressed with a constant: the
-    // compiler then sees that reading one buffer does not depend on the
-    // direct loads still filling the others, and waits only on the counter
-    __shared__ SlotT ring0[NSLOT], ring1[NSLOT], ring2[NSLOT], ring3[NSLOT];
-    static_assert(kRing == 4, "four ring buffers");
-    __shared__ uint64_t dlo[64], dhi[64];
-    __shared__ uint16_t offq[32], offg_lo[64], offg_hi[8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    if (tid < 128) {
-        const int j = tid & 63, hi = tid >> 6;
-        uint64_t o = 0;
-        for (int b = 0; b < 6; b++) {
-            const int qi = hi * 6 + b;
-            if (qi < G.tb && ((j >> b) & 1)) o |= 1ull << G.tq[qi];
-        }
-        if (hi) dhi[j] = o;
-        else dlo[j] = o;
-    } else if (tid < 160) {
-        offq[tid - 128] = G.offq[tid - 128];
-    } else if (tid < 224) {
-        offg_lo[tid - 160] = G.offg_lo[tid - 160];
-    } else if (tid < 232) {
-        offg_hi[tid - 224] = G.offg_hi[tid - 224];
-    }
-    Block<R, K> blk;
-    blk.load(mat, lane);
-    __syncthreads();
-
-    // load / store i of this thread: LDS slot 64 (wave + 4 i) + lane, tile slot swz of it
-    uint64_t off[PER];
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int l = A * ring_swz<R>(64 * (wave + 4 * i) + lane);
-        off[i] = dlo[l & 63] | dhi[l >> 6];
-    }
-    // global offsets of the result rows this lane writes
-    uint64_t gq[Block<R, K>::NOUT];
-#pragma unroll
-    for (int o = 0; o < Block<R, K>::NOUT; o++) {
-        const int lq = offq[Block<R, K>::out_row(o, lane)];
-        gq[o] = dlo[lq & 63] | dhi[lq >> 6];
-    }
-    ring_loop<R, K, GB, NT>(a, ntiles, G, ring0, ring1, ring2, ring3, offq, offg_lo, offg_hi, dlo, dhi, blk, off, gq);
-}
-
 // Tile set = Q plus the 6 + gb lowest other qubits; local indices follow the
 // ascending tile-set order.
 bool make_geom(int n, int k, int gb, const int *qubits, DenseGeom &G)
@@ -546,7 +360,6 @@ bool make_geom(int n, int k, int gb, const int *qubits, DenseGeom &G)
     const uint64_t tm = qm | lm;
     G.tb = 0;
     G.gb = gb;
-    G.plain = 0;
     int local[64];
     for (int q = 0; q < n; q++)
         if (tm & (1ull << q)) {
@@ -577,86 +390,26 @@ bool make_geom(int n, int k, int gb, const int *qubits, DenseGeom &G)
     return true;
 }
 
-// Cached launch facts are shared by every thread that launches; they are the
-// same whichever thread computes them first.
-int device_cus()
-{
-    static std::atomic<int> cached{0};
-    int cus = cached.load(std::memory_order_relaxed);
-    if (!cus) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        cached.store(cus, std::memory_order_relaxed);
-    }
-    return cus;
-}
+// Short-lived workgroups in plain order (workgroup b takes tiles b, b + grid,
+// ...): kTilesPerGroup tiles each, so the dispatcher keeps refilling the CUs.
+// A resident wave of persistent workgroups streamed HBM slower (fusion
+// workload: complex64 4.39 -> 3.97 s at four tiles per workgroup; complex128
+// 15.4-15.6 -> 14.8-15.2 s at 32, against the persistent ring kernel this
+// replaced; fewer tiles per workgroup pay the A-operand and index-table
+// set-up of every workgroup: complex128 at 2 tiles 18 % slower).
+template <typename R>
+constexpr uint64_t kTilesPerGroup = sizeof(R) == 4 ? 4 : 32;
 
-// Persistent grid: as many workgroups as the device holds at once, so no
-// workgroup waits for a slot while others still have tiles to take.
-template <typename R, int K, int GB, bool NT>
-int resident_blocks()
-{
-    static std::atomic<int> cached{0};   // one per kernel instantiation
-    int blocks = cached.load(std::memory_order_relaxed);
-    if (!blocks) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dense_mfma<R, K, GB, NT>, kThreads, 0) !=
-                hipSuccess ||
-            per_cu < 1)
-            per_cu = 1;
-        blocks = per_cu * device_cus();
-        cached.store(blocks, std::memory_order_relaxed);
-    }
-    return blocks;
-}
-
-// The ring kernel for complex128 5-qubit blocks whose qubits are all >= 3:
-// the three lowest qubits are then group bits, so the accumulator stores of
-// 8 neighbouring groups fill whole 128-byte lines.  Measured on 30-qubit
-// states (5 consecutive qubits at every position): 5.4-5.55 TB/s where the
-// staged kernel gives 4.3-5.1 TB/s for most positions (it reaches 5.6-6.1
-// for blocks starting at qubit 7..12, where the ring gives 5.5); with a block
-// on qubits 0..2 the ring's stores are partial lines (4.1-4.6 TB/s, the
-// staged kernel 5.0).  Complex64 stays on the staged kernel at every
-// position (ring 4.5-4.7 TB/s, staged 4.2-5.3).
 template <typename R, int K, int GB>
-hipError_t run_mfma(const SvDev &s, const DenseGeom &G, const void *d_mat, bool nt, bool low_groups)
+hipError_t run_mfma(const SvDev &s, const DenseGeom &G, const void *d_mat, bool nt)
 {
     const uint64_t ntiles = (1ull << s.n) >> G.tb;
-    if constexpr (sizeof(R) == 8 && K == 5 && GB == 0) {
-        if (low_groups) {
-            const int cus = device_cus();
-            const unsigned grid = (unsigned)(ntiles < (uint64_t)cus ? ntiles : (uint64_t)cus);
-            if (nt)
-                k_dense_mfma_ring<R, K, GB, true><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
-                                                                                   (const cplx_t<R> *)d_mat);
-            else
-                k_dense_mfma_ring<R, K, GB, false><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
-                                                                                    (const cplx_t<R> *)d_mat);
-            return hipGetLastError();
-        }
-    }
-    // complex128: a persistent grid (one resident wave of workgroups).
-    // complex64: about four tiles per workgroup in plain order, so workgroups
-    // are short-lived and the dispatcher keeps refilling the CUs: 32-qubit
-    // fusion workload 4.39 -> 4.04 s against the persistent grid (which
-    // complex128 keeps: its short-lived variant was 18 % slower)
-    DenseGeom g = G;
-    unsigned grid;
-    if constexpr (sizeof(R) == 4) {
-        grid = (unsigned)(ntiles >= 4 ? ntiles / 4 : 1);
-        g.plain = 1;
-    } else {
-        const int mb = nt ? resident_blocks<R, K, GB, true>() : resident_blocks<R, K, GB, false>();
-        grid = (unsigned)(ntiles < (uint64_t)mb ? ntiles : (uint64_t)mb);
-    }
+    const unsigned grid = (unsigned)(ntiles >= kTilesPerGroup<R> ? ntiles / kTilesPerGroup<R> : 1);
     if (nt)
-        k_dense_mfma<R, K, GB, true><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, g,
+        k_dense_mfma<R, K, GB, true><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
                                                                       (const cplx_t<R> *)d_mat);
     else
-        k_dense_mfma<R, K, GB, false><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, g,
+        k_dense_mfma<R, K, GB, false><<<grid, kThreads, 0, s.stream>>>((cplx_t<R> *)s.amps, ntiles, G,
                                                                        (const cplx_t<R> *)d_mat);
     return hipGetLastError();
 }
@@ -669,11 +422,9 @@ template <typename R, int K>
 hipError_t dispatch_k(const SvDev &s, const int *qubits, const void *d_mat, bool nt)
 {
     constexpr int W = wide_gb(sizeof(R) == 4, K);
-    uint32_t low = 0;
-    for (int j = 0; j < K; j++) low |= qubits[j] < 3 ? 1u : 0u;
     DenseGeom G;
-    if (make_geom(s.n, K, W, qubits, G)) return run_mfma<R, K, W>(s, G, d_mat, nt, low == 0);
-    if (make_geom(s.n, K, 0, qubits, G)) return run_mfma<R, K, 0>(s, G, d_mat, nt, low == 0);
+    if (make_geom(s.n, K, W, qubits, G)) return run_mfma<R, K, W>(s, G, d_mat, nt);
+    if (make_geom(s.n, K, 0, qubits, G)) return run_mfma<R, K, 0>(s, G, d_mat, nt);
     return hipErrorInvalidValue;
 }
 
