@@ -134,7 +134,7 @@ __device__ __forceinline__ R pick4(int k, R a, R b, R c, R d)
     const R dr2 = wave_uniform((R)(op).m[4]), di2 = wave_uniform((R)(op).m[5]);                           \
     const R dr3 = wave_uniform((R)(op).m[6]), di3 = wave_uniform((R)(op).m[7])
 
-// The tiles a persistent workgroup takes: t, t + step, ... below end.  The
+// The tiles a workgroup takes: t, t + step, ... below end.  The
 // dispatcher hands consecutive workgroups to the 8 XCDs in turn, so when the
 // grid and the tile count split evenly, workgroup b (on XCD b % 8) walks the
 // b % 8-th eighth of the tiles and each XCD streams one contiguous range;
