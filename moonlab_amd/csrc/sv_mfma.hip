@@ -278,21 +278,27 @@ __global__ __launch_bounds__(kThreads) void k_dense_mfma(cplx_t<R> *__restrict__
     __shared__ uint16_t offq[32], offg_lo[64], offg_hi[8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    if (tid < 128) {
-        const int j = tid & 63, hi = tid >> 6;
+    // Index tables.  The tile set is read with constant indices (scalar
+    // loads) and every per-thread load is issued before any is used: a
+    // workgroup lives for a few tiles only, and the former loop over a
+    // thread-dependent index fetched each qubit byte with its own wait.
+    {
+        const int j = tid & 63;
+        const bool upper = tid >= 64;   // threads 64..127: tile bits 6..11
         uint64_t o = 0;
+#pragma unroll
         for (int b = 0; b < 6; b++) {
-            const int qi = hi * 6 + b;
-            if (qi < G.tb && ((j >> b) & 1)) o |= 1ull << G.tq[qi];
+            const int qlo = G.tq[b], qhi = G.tq[6 + b];
+            const bool on = (j >> b) & 1;
+            if (on && !upper && b < G.tb) o |= 1ull << qlo;
+            if (on && upper && 6 + b < G.tb) o |= 1ull << qhi;
         }
-        if (hi) dhi[j] = o;
-        else dlo[j] = o;
-    } else if (tid < 160) {
-        offq[tid - 128] = G.offq[tid - 128];
-    } else if (tid < 224) {
-        offg_lo[tid - 160] = G.offg_lo[tid - 160];
-    } else if (tid < 232) {
-        offg_hi[tid - 224] = G.offg_hi[tid - 224];
+        const uint16_t q = G.offq[tid & 31], gl = G.offg_lo[tid & 63], gh = G.offg_hi[tid & 7];
+        if (tid < 64) dlo[j] = o;
+        else if (tid < 128) dhi[j] = o;
+        if (tid < 32) offq[tid] = q;
+        else if (tid >= 128 && tid < 192) offg_lo[j] = gl;
+        else if (tid >= 192 && tid < 200) offg_hi[tid - 192] = gh;
     }
 
     Block<R, K> blk;
