@@ -117,6 +117,20 @@ def test_binding_declares_every_symbol_it_calls():
         assert getattr(L, name).argtypes is not None, name
 
 
+def test_hea_pack_layout():
+    """HeaPack: descriptors point into one flat angle buffer at each
+    instance's offset, in the layout moonlab_mi355x_hea_t declares."""
+    from moonlab_amd import workloads
+    inst = workloads.hea_batch(40, 7)
+    p = m.HeaPack(inst)
+    assert p.count == 40 and p.angles.dtype == np.float64
+    for i, (n, L, ang, h) in enumerate(inst):
+        assert p._arr["num_qubits"][i] == n and p._arr["num_layers"][i] == L and p._arr["h"][i] == h
+        assert p._arr["angles"][i] == p.angles.ctypes.data + 8 * int(p.offsets[i])
+        assert np.array_equal(p.angles[p.offsets[i]:p.offsets[i] + 2 * n * L], np.asarray(ang).reshape(-1))
+    assert p.offsets[-1] + 2 * inst[-1][0] * inst[-1][1] == p.angles.size
+
+
 def test_job_introspection_and_errors():
     L = m.lib()
     assert not L.moonlab_job_create(0)

@@ -37,6 +37,25 @@ def test_hea_tfim_golden(eng):
     assert np.max(np.abs(e - want)) < TOL
 
 
+def test_hea_pack_matches_instance_list(eng):
+    """A HeaPack gives the energies and gradients of the same instances
+    passed as a list, bit for bit, and angles written into it in place take
+    effect on the next call."""
+    from moonlab_amd import workloads
+    inst = workloads.hea_batch(48, 31)
+    rng = np.random.default_rng(32)
+    inst2 = [(n, L, rng.uniform(0, 2 * np.pi, (L, n, 2)), h) for (n, L, _, h) in inst]
+    b = eng.Batch()
+    pack = eng.HeaPack(inst)
+    assert np.array_equal(b.hea_tfim(pack), b.hea_tfim(inst))
+    pack.angles[:] = np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1) for (_, _, a, _) in inst2])
+    assert np.array_equal(b.hea_tfim(pack), b.hea_tfim(inst2))
+    small = inst[:6]
+    e1, g1 = b.hea_tfim_gradients(eng.HeaPack(small))
+    e2, g2 = b.hea_tfim_gradients(small)
+    assert np.array_equal(e1, e2) and all(np.array_equal(x, y) for x, y in zip(g1, g2))
+
+
 def test_hea_tfim_cache_blocked_vs_oracle(eng, oracle):
     """n = 13..18 runs from HBM slots through LDS tiles; n <= 12 stays in LDS."""
     rng = np.random.default_rng(31)
