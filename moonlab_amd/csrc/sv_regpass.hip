@@ -459,33 +459,23 @@ __global__ __launch_bounds__(1 << (TB - RB), WPE) void k_reg_pass(cplx_t<R> *__r
             // each op's head is loaded while the op before it runs: its
             // address depends on the previous head's arm (after()), so the
             // load issues only once that head has arrived, and the wait
-            // before an op never includes the load of the op after it
-            // (complex64 ops are short enough that the prefetch's extra
-            // registers cost more than the latency it hides: measured 10 %
-            // slower, so complex64 loads each head where it is used)
+            // before an op never includes the load of the op after it.  The
+            // op after a phase's last op is the next phase's header (or the
+            // spare op the host puts after the last op): fetched, never run.
+            // (For complex64 this prefetch was once 10 % slower than loading
+            // each head where it is used; on short-lived workgroups it is
+            // 0.5 % faster, so both precisions share it.)
             cplx_t<R> w[NV];
-            const BOp *po = pops + k;
-            if constexpr (sizeof(R) == 8) {
-                // the op after a phase's last op is the next phase's header
-                // (or the spare op the host puts after the last op): fetched,
-                // never run
-                const BOp *p = po + 1;
-                OpHead h0 = op_head(p);
-                for (int q = 1; q <= cnt; q += 2, p += 2) {
-                    const OpHead h1 = op_head(after(p + 1, h0.arm));
-                    pp_op<R, RB>(v, w, h0, p, base);
-                    if (q < cnt) {
-                        h0 = op_head(after(p + 2, h1.arm));
-                        pp_op<R, RB>(w, v, h1, p + 1, base);
-                    } else {
-                        pp_copy<R, RB>(w, v);
-                    }
-                }
-            } else {
-                for (int q = 1; q <= cnt; q += 2) {
-                    pp_op<R, RB>(v, w, op_head(po + q), po + q, base);
-                    if (q < cnt) pp_op<R, RB>(w, v, op_head(po + q + 1), po + q + 1, base);
-                    else pp_copy<R, RB>(w, v);
+            const BOp *p = pops + k + 1;
+            OpHead h0 = op_head(p);
+            for (int q = 1; q <= cnt; q += 2, p += 2) {
+                const OpHead h1 = op_head(after(p + 1, h0.arm));
+                pp_op<R, RB>(v, w, h0, p, base);
+                if (q < cnt) {
+                    h0 = op_head(after(p + 2, h1.arm));
+                    pp_op<R, RB>(w, v, h1, p + 1, base);
+                } else {
+                    pp_copy<R, RB>(w, v);
                 }
             }
             k += 1 + cnt;
